@@ -1,0 +1,177 @@
+/*
+ * mftx — MI355X matrix Fourier transform library (C ABI).
+ *
+ * The drop-in boundary of the lentil Fraunhofer path.  The reference is pure
+ * Python + NumPy and has no FFI, so each entry point here replaces the NumPy
+ * call sites of one reference function; the Python package lentil_amd binds
+ * them with ctypes and keeps lentil's own signatures above them.
+ *
+ * Conventions
+ *   - every function returns 0 on success or a negative MFTX_E* code; the
+ *     message of the last failure on a context is mftx_last_error(ctx);
+ *   - all array arguments are device pointers, row-major, with explicit row
+ *     strides in elements; the caller owns every buffer it passes;
+ *   - work is queued on the context's stream (mftx_set_stream) and runs
+ *     asynchronously; the library allocates only context- or plan-owned
+ *     workspace, never a caller buffer;
+ *   - dtype codes: 0 complex64, 1 float32, 2 complex128, 3 float64;
+ *   - precision codes: 0 FP32 (complex64 operands, FP32 MFMA),
+ *     1 BF16X3 (split hi/lo BF16 operands, three MFMA passes, FP32 accumulate).
+ */
+#ifndef MFTX_H
+#define MFTX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFTX_OK 0
+#define MFTX_EINVAL -1   /* bad argument: shape, dtype, pointer (maps to ValueError/TypeError) */
+#define MFTX_ENOMEM -2   /* workspace allocation failed */
+#define MFTX_EDEVICE -3  /* HIP runtime error */
+#define MFTX_EHANDLE -4  /* null or destroyed handle */
+
+#define MFTX_DTYPE_C64 0
+#define MFTX_DTYPE_F32 1
+#define MFTX_DTYPE_C128 2
+#define MFTX_DTYPE_F64 3
+
+#define MFTX_PREC_FP32 0
+#define MFTX_PREC_BF16X3 1
+
+typedef struct mftx_ctx mftx_ctx;
+typedef struct mftx_plan mftx_plan;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mftx_version(void);
+
+/* Context: owns a device, a stream binding and a growable workspace.
+ * No counterpart in the reference (NumPy needs no context). */
+int mftx_create(int device, mftx_ctx** out);
+int mftx_destroy(mftx_ctx* ctx);
+const char* mftx_last_error(const mftx_ctx* ctx);
+/* Queue subsequent work on `stream` (a hipStream_t; NULL = default stream). */
+int mftx_set_stream(mftx_ctx* ctx, void* stream);
+/* Block until the context's stream has drained. */
+int mftx_synchronize(mftx_ctx* ctx);
+
+/*
+ * One matrix Fourier transform:
+ *   F[u][v] = scale * sum_{r,s} f[r][s] * exp(sgn*2*pi*i*(ar*(R_r+or)*(U_u-sr) + ac*(S_s+oc)*(V_v-sc)))
+ * with R_r = r - floor(m/2), S_s = s - floor(n/2), U_u = u - floor(M/2),
+ * V_v = v - floor(N/2), sgn = -1 (conj == 0) or +1 (conj == 1).
+ *
+ * Replaces lentil fourier.dft2 (fourier.py:5-103): twiddles of
+ * _dft2_matrices (fourier.py:106-110), the two np.dot calls (fourier.py:97)
+ * and the unitary scale (fourier.py:100-101, passed in `scale`).  idft2
+ * (fourier.py:124-198) is conj == 1 with scale = unitary/F.size; the adjoint
+ * of dft2 is conj == 1 with the roles of (shift, offset) swapped and negated.
+ *
+ * f: m x n, dtype f_dtype (any of the four codes), row stride ldf.
+ * F: M x N complex64, row stride ldF.  F must not overlap f.
+ */
+int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, int64_t ldf,
+              void* F, int64_t M, int64_t N, int64_t ldF,
+              double alpha_r, double alpha_c, double shift_r, double shift_c,
+              double offset_r, double offset_c, double scale, int conj, int precision);
+
+/*
+ * Pupil phasor  out = amp * exp(2*pi*i*opd/wavelength)  (complex64, m x n, dense).
+ * Replaces the phasor of Plane.__mul__ (plane.py:503-510).  amp and opd are
+ * float32 or float64 (dtype codes 1 or 3) with row strides ld_amp, ld_opd.
+ */
+int mftx_phasor(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp,
+                const void* opd, int opd_dtype, int64_t ld_opd, int64_t m, int64_t n,
+                double wavelength, void* out);
+
+/*
+ * out[or0 + r][oc0 + c] += weight * |E|^2   (intensity != 0, out float32/float64)
+ * out[or0 + r][oc0 + c] += weight * E       (intensity == 0, out complex64/complex128)
+ * for E = field[fr0 + r][fc0 + c], 0 <= r < h, 0 <= c < w.
+ * Replaces the accumulation of field.insert (field.py:301-304); the caller
+ * passes the clipped slices of field.py:259-299.  Coherent merging of
+ * overlapping Fields (field._merge, field.py:339-342) is the complex form
+ * into a zeroed union buffer.
+ */
+int mftx_insert(mftx_ctx* ctx, const void* field, int64_t ldf, int64_t fr0, int64_t fc0,
+                int64_t h, int64_t w, void* out, int out_dtype, int64_t ldo,
+                int64_t or0, int64_t oc0, int intensity, double weight);
+
+/*
+ * out = a * b elementwise over h x w complex64 arrays (row strides lda, ldb, ldo).
+ * Replaces the Field product of Field.__mul__ (field.py:142) on the
+ * intersection slices computed by the host (field.py:130-147).
+ */
+int mftx_multiply(mftx_ctx* ctx, const void* a, int64_t lda, const void* b, int64_t ldb,
+                  void* out, int64_t ldo, int64_t h, int64_t w);
+
+/* Fill `bytes` bytes at device pointer p with zeros (a kernel, not a memset node). */
+int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes);
+
+/* ---------------------------------------------------------------------------
+ * Batched broadband / segmented PSF plan.
+ *
+ * Defined as the reference user loop (docs/user/diffraction.rst:148-160):
+ *   for each wavelength w:  W = Wavefront(w) * pupil [* tilts];
+ *                           W = propagate_dft(W, ...);  image = W.insert(image, weight_w)
+ * i.e. Plane.__mul__ (plane.py:477-516) -> propagate_dft (propagate.py:147-242)
+ * -> Wavefront.insert (wavefront.py:145-165) with coherent merging of
+ * overlapping Fields (field.py:413-461).  The host computes the per-Field
+ * geometry of propagate_dft and passes one mftx_problem per (tile, wavelength)
+ * that reaches the output; the plan runs every problem in a few grouped
+ * launches and accumulates weight*|E|^2 onto the detector image.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+  const void* amp;   /* m x n tile amplitude (already multiplied by its segment mask) */
+  const void* opd;   /* m x n tile OPD in metres */
+  int32_t amp_dtype, opd_dtype;  /* 1 float32 or 3 float64 */
+  int64_t ld_amp, ld_opd;
+  int64_t m, n;
+} mftx_tile;
+
+typedef struct {
+  int32_t tile;        /* index into tiles */
+  int32_t wave;        /* index into wavelengths */
+  int32_t group;       /* coherent group: problems with equal group are summed as fields before |.|^2 */
+  int32_t M, N;        /* dft2 output shape (propagate.py:215) */
+  double alpha_r, alpha_c;     /* propagate.py:232-234 */
+  double shift_r, shift_c;     /* prop_shift + subpixel shift (propagate.py:235-236) */
+  double offset_r, offset_c;   /* the Field offset (propagate.py:236) */
+  int64_t out_r0, out_c0;      /* detector index of F[0][0] (may be outside the image) */
+  double weight;               /* intensity weight of the wavelength */
+} mftx_problem;
+
+typedef struct {
+  double flops;          /* algorithmic FLOPs of one execute (8 per complex MAC) */
+  double gemm_flops[2];  /* per GEMM stage */
+  double bytes_hbm_min;  /* minimum HBM bytes of one execute (phasor + intermediates + image) */
+  int32_t n_launch;      /* kernel launches per execute */
+  int32_t coherent_fast; /* 1 if every coherent group runs as one K-concatenated GEMM */
+  int32_t n_stage1, n_stage2, n_work1, n_work2;
+} mftx_plan_info;
+
+/* Build a plan; validates every problem (MFTX_EINVAL on a bad index or shape)
+ * and allocates plan-owned workspace.  Tile pointers are kept and re-read on
+ * every execute, so amp/opd contents may change between executes. */
+int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles,
+                  const double* wavelengths, int32_t nwave,
+                  const mftx_problem* problems, int32_t nprob,
+                  int64_t image_rows, int64_t image_cols, int32_t precision,
+                  int32_t lambda_group, mftx_plan** out);
+/* image: device float32 (dtype 1) or float64 (dtype 3), image_rows x image_cols,
+ * row stride ldi.  accumulate != 0 adds to image (Wavefront.insert), else overwrites. */
+int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi,
+                      int32_t accumulate);
+/* Capture the plan's launches in a hipGraph for this image pointer and replay it
+ * on later executes with the same pointer (use_graph != 0), or run eagerly. */
+int mftx_plan_set_graph(mftx_plan* plan, int32_t use_graph);
+int mftx_plan_get_info(const mftx_plan* plan, mftx_plan_info* info);
+int mftx_plan_destroy(mftx_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFTX_H */

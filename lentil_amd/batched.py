@@ -1,0 +1,210 @@
+"""Batched broadband / segmented PSFs on one MI355X.
+
+``PSFPlan`` runs the reference user loop (docs/user/diffraction.rst:148-160)
+
+    img = zeros(shape * oversample)
+    for wl, wt in zip(wavelengths, weights):
+        w = Wavefront(wl) * pupil            # one phasor Field per segment
+        for t in tilts: w = w * t
+        w = propagate_dft(w, pixelscale, shape, prop_shape, oversample, mask)
+        img = w.insert(img, weight=wt)       # coherent merge, then |E|^2
+
+as a handful of grouped launches (phasors for every (segment, wavelength),
+one stage-1 GEMM launch, one stage-2 GEMM launch with |E|^2 accumulated in
+registers across wavelengths, one reduction onto the detector), optionally
+replayed from a hipGraph.  The per-Field geometry is the same host code as
+``propagate_dft`` (``propagate.field_geometry``), so both paths agree on
+every shift, clip and offset.
+"""
+import ctypes
+from itertools import combinations
+
+import numpy as np
+import torch
+
+from . import _lib, extent as _ext, helper
+from .field import extents_boundary, tilt_shift
+from .plane import Pupil, _mul_pixelscale
+from .propagate import field_geometry, output_extent
+
+
+class _TileField:
+    """Geometry of a phasor Field: offset and tilt list (what propagate_dft reads)."""
+
+    def __init__(self, shape, offset, tilt):
+        self.shape = shape
+        self.offset = offset
+        self.tilt = tilt
+
+    def shift(self, z, wavelength, pixelscale, oversample, indexing='ij'):
+        return tilt_shift(self.tilt, z, wavelength, pixelscale, oversample, indexing)
+
+
+def coherent_groups(extents):
+    """Indices of transitively overlapping extents, grouped in the reference's
+    join-first-pair-and-rescan order (field.py:451-461, field.boundary quirks
+    included)."""
+    groups = [([i], e) for i, e in enumerate(extents)]
+    joined = True
+    while joined:
+        joined = False
+        for a, b in combinations(range(len(groups)), 2):
+            if _ext.intersect(groups[a][1], groups[b][1]):
+                idx = groups[a][0] + groups[b][0]
+                groups[a] = (idx, extents_boundary([extents[i] for i in idx]))
+                groups.pop(b)
+                joined = True
+                break
+    return [g[0] for g in groups]
+
+
+def plan_geometry(pupil, wavelengths, weights=None, pixelscale=None, shape=None, prop_shape=None, oversample=2,
+                  mask=None, tilts=()):
+    """Host bookkeeping of the reference loop for every (segment, wavelength).
+
+    Returns dict(shape_out, wavelengths, weights, tiles=[(amp, opd)],
+    problems=[dict(tile, wave, group, M, N, alpha, shift, offset, out_offset,
+    out_r0, out_c0, weight)]) where ``group`` numbers the coherent groups that
+    Wavefront.insert would merge before squaring (field.py:413-461) and
+    (out_r0, out_c0) is where F[0][0] lands on the detector
+    (field.py:267, before clipping).
+    """
+    if not isinstance(pupil, Pupil):
+        raise TypeError('PSFPlan needs a Pupil')
+    if pupil.focal_length is None:
+        raise ValueError('pupil focal_length must be set')
+    if pixelscale is None:
+        raise ValueError('pixelscale must be given')
+    wavelengths = np.atleast_1d(np.asarray(wavelengths, dtype=float))
+    if weights is None:
+        weights = np.ones_like(wavelengths)
+    weights = np.broadcast_to(np.asarray(weights, dtype=float), wavelengths.shape).copy()
+    dx = _mul_pixelscale(pupil.pixelscale, None)
+    if dx is None:
+        raise ValueError('pupil pixelscale must be set')
+    du = np.broadcast_to(pixelscale, (2,))
+    z = pupil.focal_length
+    shape = np.asarray(pupil.shape) if shape is None else np.broadcast_to(shape, (2,))
+    prop_shape = np.asarray(shape) if prop_shape is None else np.broadcast_to(prop_shape, (2,))
+    shape_out, out_ext = output_extent(shape, oversample, mask)
+    shape_out = tuple(int(v) for v in shape_out)
+    prop_out = prop_shape * oversample
+
+    tiles, tile_fields = [], []
+    for n, s, amp, opd in pupil.phasor_tiles():
+        amp = np.asarray(amp)
+        opd = np.asarray(opd)
+        if amp.size == 1 and opd.size == 1:
+            raise ValueError('PSFPlan needs an array amplitude or OPD')
+        if np.iscomplexobj(amp):
+            raise TypeError('PSFPlan supports real amplitudes')
+        shp = opd.shape if amp.size == 1 else amp.shape
+        tiles.append((np.broadcast_to(amp, shp), np.broadcast_to(opd, shp)))
+        tilt = ([pupil.tilt[n]] if pupil.tilt else []) + list(tilts)
+        tile_fields.append(_TileField(shp, helper.slice_offset(s, pupil.shape), tilt))
+
+    problems = []
+    gid = 0
+    for w, (wl, wt) in enumerate(zip(wavelengths, weights)):
+        geo = []
+        for t, tf in enumerate(tile_fields):
+            g = field_geometry(tf, wl, dx, du, z, prop_out, out_ext, oversample)
+            if g is not None:
+                geo.append((t, g))
+        for members in coherent_groups([_ext.array_extent(g['shape'], g['out_offset']) for _, g in geo]):
+            for i in members:
+                t, g = geo[i]
+                M, N = (int(v) for v in g['shape'])
+                problems.append(dict(
+                    tile=t, wave=w, group=gid, M=M, N=N, alpha=tuple(float(a) for a in g['alpha']),
+                    shift=tuple(float(v) for v in g['shift']), offset=tuple(float(v) for v in g['offset']),
+                    out_offset=tuple(int(v) for v in g['out_offset']),
+                    out_r0=shape_out[0] // 2 - M // 2 + int(g['out_offset'][0]),
+                    out_c0=shape_out[1] // 2 - N // 2 + int(g['out_offset'][1]), weight=float(wt)))
+            gid += 1
+    return dict(shape_out=shape_out, wavelengths=wavelengths, weights=weights, tiles=tiles, problems=problems)
+
+
+class PSFPlan:
+    """Weighted broadband (and segmented) PSF accumulated on the device.
+
+    Parameters mirror the reference loop: ``pupil`` is a lentil_amd
+    ``Pupil`` (2-D mask: one tile; 3-D mask: one tile per segment), ``tilts``
+    are Tilt/DispersiveTilt planes applied after the pupil.  ``precision`` is
+    'fp32' or 'bf16x3'; ``lambda_group`` wavelengths share one output tile
+    accumulation in registers; ``use_graph`` replays the launches from a
+    hipGraph after the first call.
+    """
+
+    def __init__(self, pupil, wavelengths, weights=None, pixelscale=None, shape=None, prop_shape=None,
+                 oversample=2, mask=None, tilts=(), precision='bf16x3', lambda_group=4, use_graph=True):
+        geo = plan_geometry(pupil, wavelengths, weights, pixelscale, shape, prop_shape, oversample, mask, tilts)
+        self.wavelengths = geo['wavelengths']
+        self.weights = geo['weights']
+        self.precision = precision
+        self.oversample = oversample
+        self.shape_out = geo['shape_out']
+        self.tile_shapes = [a.shape for a, _ in geo['tiles']]
+
+        # device tiles (kept alive: the plan holds raw pointers)
+        dev = _lib.device()
+        self._amp_opd = []
+        tiles = []
+        for amp, opd in geo['tiles']:
+            ta = torch.from_numpy(np.ascontiguousarray(amp, dtype=np.float32)).to(dev)
+            to = torch.from_numpy(np.ascontiguousarray(opd, dtype=np.float64)).to(dev)
+            self._amp_opd.append((ta, to))
+            tiles.append(_lib.MftxTile(ta.data_ptr(), to.data_ptr(), _lib.DT_F32, _lib.DT_F64, ta.stride(0),
+                                       to.stride(0), amp.shape[0], amp.shape[1]))
+        problems = [_lib.MftxProblem(p['tile'], p['wave'], p['group'], p['M'], p['N'], p['alpha'][0],
+                                     p['alpha'][1], p['shift'][0], p['shift'][1], p['offset'][0], p['offset'][1],
+                                     p['out_r0'], p['out_c0'], p['weight']) for p in geo['problems']]
+        self.n_problems = len(problems)
+        self.problems = geo['problems']
+        ctx = _lib.context()
+        lib = _lib.load()
+        tarr = (_lib.MftxTile * len(tiles))(*tiles)
+        warr = (ctypes.c_double * len(self.wavelengths))(*self.wavelengths)
+        parr = (_lib.MftxProblem * max(len(problems), 1))(*problems)
+        h = ctypes.c_void_p()
+        rc = lib.mftx_plan_psf(ctx, tarr, len(tiles), warr, len(self.wavelengths), parr, len(problems),
+                               self.shape_out[0], self.shape_out[1], _lib.precision_code(precision),
+                               int(lambda_group), ctypes.byref(h))
+        _lib.check(rc, ctx, 'mftx_plan_psf')
+        self._plan = h.value
+        lib.mftx_plan_set_graph(self._plan, 1 if use_graph else 0)
+        info = _lib.MftxPlanInfo()
+        lib.mftx_plan_get_info(self._plan, ctypes.byref(info))
+        self.info = {k: (list(getattr(info, k)) if k == 'gemm_flops' else getattr(info, k))
+                     for k, _ in _lib.MftxPlanInfo._fields_}
+
+    def __call__(self, out=None, accumulate=False):
+        """Compute the image; returns ``out`` (a float32/float64 device tensor)."""
+        if out is None:
+            out = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
+            accumulate = False
+        if tuple(out.shape) != self.shape_out or out.dim() != 2 or out.stride(1) != 1:
+            raise ValueError(f'out must be a {self.shape_out} row-major device array')
+        code = _lib.dtype_code(out)
+        if code not in (_lib.DT_F32, _lib.DT_F64):
+            raise TypeError('out must be float32 or float64')
+        ctx = _lib.context()
+        rc = _lib.load().mftx_plan_execute(self._plan, ctypes.c_void_p(out.data_ptr()), code, out.stride(0),
+                                           1 if accumulate else 0)
+        _lib.check(rc, ctx, 'mftx_plan_execute')
+        return out
+
+    def set_opd(self, tile, opd):
+        """Overwrite a tile's OPD in place (the plan re-reads it on every call)."""
+        self._amp_opd[tile][1].copy_(torch.as_tensor(opd, dtype=torch.float64))
+
+    def close(self):
+        if getattr(self, '_plan', None):
+            _lib.load().mftx_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,90 @@
+// Internal declarations shared by the MFT kernels and the C-ABI host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mftx {
+
+// Tile geometry of the complex GEMM C = A . B (A data, B generated twiddles).
+constexpr int BM = 128;   // C rows per workgroup (A rows)
+constexpr int BN = 128;   // C cols per workgroup (twiddle columns)
+constexpr int BK = 16;    // K per pipeline stage
+constexpr int NTHREADS = 256;
+
+enum Precision { PREC_FP32 = 0, PREC_BF16X3 = 1 };
+
+// Epilogues of the GEMM.
+enum Epilogue {
+  EPI_STORE_CPLX = 0,   // dst complex64, (r, c) -> dst[c*ld + r] if trans else dst[r*ld + c]; * scale; bounds
+  EPI_STORE_PACKED_T = 1,  // dst packed A-operand layout (precision dependent) of C^T, rows r < limit
+  EPI_INTENSITY = 2,    // accumulate w*|C|^2 over the workgroup's problems into a partial image
+};
+
+// One complex GEMM problem.  Twiddle B[k][j] = cis(alpha * x_k * y_j) with
+// x_k = seg.x0 + (k - seg.kbeg) and y_j = y0 + j; cis(t) = exp(2*pi*i*t).
+struct Problem {
+  const void* a;        // packed A operand (row-major, K-blocked)
+  long long lda;        // A row stride in scalar elements (float2 for fp32, bf16 for bf16x3)
+  double alpha;         // signed twiddle scale in turns per unit of x*y
+  double y0;            // y of column 0
+  int rows, cols;       // valid C extent
+  int kpad;             // K (multiple of BK); A is zero beyond the valid K
+  int seg_begin, seg_count;
+  float scale;          // store: C *= scale; intensity: weight applied to |C|^2
+  int limit;            // EPI_STORE_PACKED_T: C rows written (K extent of the destination)
+  void* dst;            // store destination (EPI_STORE_*)
+  long long ldd;        // destination row stride (elements of its type)
+  int trans;            // EPI_STORE_CPLX / EPI_INTENSITY: write C transposed
+  int pad_;
+};
+
+struct Segment {
+  int kbeg, kend;       // [kbeg, kend), multiples of BK
+  double x0;            // x at k = kbeg
+};
+
+// A workgroup's unit of work: problems [pbeg, pend) on tile (tr, tc) of C.
+// For EPI_INTENSITY the problems' |C|^2 are summed into partial[r][c]
+// (row-major, stride ldp, transposed like the problems' trans flag).
+struct Work {
+  int pbeg, pend;
+  int tr, tc;
+  float* partial;
+  long long ldp;
+  int prows, pcols;     // partial image extent (bounds for the store)
+};
+
+// Packed A-operand layouts.
+//  fp32   : complex64 row-major, element (r, k) at r*lda + k.
+//  bf16x3 : per row, K in blocks of 16; each block holds 4 planes of 16 bf16
+//           (re_hi, re_lo, im_hi, im_lo): element (r, k, p) at
+//           r*lda + (k/16)*64 + p*16 + (k%16), lda = 4*kpad.
+__host__ __device__ inline long long packed_lda(int prec, int kpad) { return prec == PREC_FP32 ? kpad : 4LL * kpad; }
+__host__ __device__ inline size_t packed_bytes(int prec, long long rows, int kpad) {
+  return (size_t)rows * (size_t)kpad * (prec == PREC_FP32 ? 8 : 8);
+}
+
+// launchers (mftx_kernels.hip)
+hipError_t launch_gemm(int prec, int epi, const Work* work, int nwork, const Problem* probs,
+                       const Segment* segs, hipStream_t s);
+hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n,
+                       long long lds, void* dst, long long rows_pad, int kpad, hipStream_t s);
+hipError_t launch_phasor(int prec, const void* amp, int amp_dtype, long long ld_amp,
+                         const void* opd, int opd_dtype, long long ld_opd, int m, int n,
+                         const double* inv_wavelengths_dev, int nwave, void* dst,
+                         long long dst_wave_stride, long long rows_pad, int kpad, hipStream_t s);
+hipError_t launch_phasor_field(const void* amp, int amp_dtype, long long ld_amp, const void* opd,
+                               int opd_dtype, long long ld_opd, int m, int n, double inv_wavelength,
+                               void* dst, hipStream_t s);
+hipError_t launch_reduce_partials(const float* partial, long long group_stride, int ngroups,
+                                  int rows, int cols, long long ldp, void* out, int out_dtype,
+                                  long long ldo, int out_rows, int out_cols, int r0, int c0,
+                                  int accumulate, hipStream_t s);
+hipError_t launch_insert(const void* field, long long ldf, int fr0, int fc0, int h, int w,
+                         void* out, int out_dtype, long long ldo, int or0, int oc0,
+                         int intensity, double weight, hipStream_t s);
+hipError_t launch_multiply(const void* a, long long lda, const void* b, long long ldb, void* out,
+                           long long ldo, int h, int w, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace mftx
