@@ -1,0 +1,177 @@
+"""Throughput of the broadband MFT path on one MI355X.
+
+Workload (broadband_512): 512^2 pupil with a 12% central obscuration, Noll
+2-22 OPD at 30 nm, 64 wavelengths 500-900 nm with flat weights, 5 um pixels,
+npix 256, oversample 4 -> a 1024^2 detector image.  One step = one
+64-wavelength image (64 PSFs) through PSFPlan: phasors, stage-1 GEMM,
+stage-2 GEMM with |E|^2 accumulated over wavelengths, detector reduction.
+Inputs are resident in HBM before the timed region.
+
+Printed: one JSON line with PSFs/s, the roofline of the dominant kernel
+(stage-2 GEMM, timed with HIP events on its stream), the relative-L2 error
+of the image against the complex128 reference loop, the plain per-wavelength
+FP32 GPU path and the CPU baseline (the complex128 oracle of the reference
+loop on this host's cores).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x3|fp32]
+                    [--no-cpu] [--no-plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {'bf16x3': 2.5e15, 'fp32': 157.3e12}   # dense MFMA peaks of MI355X (FLOP/s)
+MFMA_PASSES = {'bf16x3': 3, 'fp32': 1}        # executed MFMA work / algorithmic work
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--precision', default='bf16x3', choices=['bf16x3', 'fp32'])
+    ap.add_argument('--lambda-group', type=int, default=4)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the FP64 error check')
+    ap.add_argument('--no-plain', action='store_true', help='skip the plain per-wavelength FP32 path')
+    return ap.parse_args()
+
+
+def cpu_baseline(cfg, budget_s=25.0):
+    """The complex128 reference loop (oracle) on this host's cores; returns
+    (PSFs/s, the image, threads, sample description)."""
+    from oracle import optics
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([t.get('num_threads', 1) for t in threadpool_info() if t.get('user_api') == 'blas'] or [1])
+    except Exception:
+        threads = os.cpu_count()
+    p = optics.Pupil(cfg['amp'], cfg['opd'], cfg['dx'], cfg['focal_length'])
+    wl, wt = cfg['wavelengths'], cfg['weights']
+    optics.broadband_image(p, wl[:1], wt[:1], cfg['pixelscale'], cfg['shape'], cfg['oversample'])  # warm-up
+    times, img = [], None
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        img = optics.broadband_image(p, wl, wt, cfg['pixelscale'], cfg['shape'], cfg['oversample'])
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 5 or time.perf_counter() - t_start > budget_s:
+            break
+    med = float(np.median(times))
+    return len(wl) / med, img, threads, f'full {len(wl)}-wavelength image, median of {len(times)} runs ({med:.2f} s each)'
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 '
+                                   '(multi-GPU sharding is described in DESIGN.md, not built)'}))
+        sys.exit(2)
+    import torch
+    import lentil_amd as L
+    from lentil_amd import workloads
+
+    assert torch.cuda.is_available(), 'bench needs a HIP device'
+    cfg = workloads.broadband_512()
+    nwave = len(cfg['wavelengths'])
+    pupil = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'], focal_length=cfg['focal_length'])
+    plan = L.PSFPlan(pupil, cfg['wavelengths'], cfg['weights'], pixelscale=cfg['pixelscale'], shape=cfg['shape'],
+                     oversample=cfg['oversample'], precision=args.precision, lambda_group=args.lambda_group,
+                     use_graph=True)
+    out = torch.empty(plan.shape_out, dtype=torch.float32, device='cuda')
+    for _ in range(args.warmup):
+        plan(out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan(out)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    psfs_per_s = nwave * args.steps / dt
+    image = out.double().cpu().numpy()
+    print(f'[bench] {args.precision}: {ms_step:.3f} ms/image, {psfs_per_s:.1f} PSFs/s', file=sys.stderr, flush=True)
+
+    # dominant kernel: stage-2 GEMM, HIP events on the plan's stream
+    stage_ms = plan.profile(torch.empty_like(out), repeats=5)
+    flops2 = plan.info['gemm_flops'][1]
+    ach = flops2 / (stage_ms['stage2'] * 1e-3)
+    total_flops = plan.info['flops']
+    roofline = {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK[args.precision] / 1e12, 'unit': 'TFLOP/s',
+                'frac': ach / PEAK[args.precision], 'traffic': None,
+                'kernel': 'gemm_kernel<bf16x3|fp32, intensity> (stage 2)',
+                'algorithmic_flops_per_launch': flops2, 'avg_launch_ms': stage_ms['stage2'],
+                'mfma_pipe_frac': ach * MFMA_PASSES[args.precision] / PEAK[args.precision]}
+    print(f'[bench] stages ms {stage_ms}', file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_plain:
+        # plain per-wavelength FP32 path: the reference user loop through the drop-in API, device-resident image
+        img_dev = torch.zeros(plan.shape_out, dtype=torch.float64, device='cuda')
+        for rep in range(2):
+            img_dev.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for wl, wt in zip(cfg['wavelengths'], cfg['weights']):
+                w = L.Wavefront(wl) * pupil
+                w = L.propagate_dft(w, cfg['pixelscale'], shape=cfg['shape'], oversample=cfg['oversample'],
+                                    precision='fp32')
+                w.insert(img_dev, weight=wt)
+            torch.cuda.synchronize()
+            dt_plain = time.perf_counter() - t0
+        extra['plain_fp32_per_wavelength'] = {'psfs_per_s': nwave / dt_plain, 'ms_per_image': 1e3 * dt_plain}
+        extra['speedup_vs_plain_fp32'] = psfs_per_s / (nwave / dt_plain)
+        plain_img = img_dev.cpu().numpy()
+    if args.precision != 'fp32':
+        p32 = L.PSFPlan(pupil, cfg['wavelengths'], cfg['weights'], pixelscale=cfg['pixelscale'], shape=cfg['shape'],
+                        oversample=cfg['oversample'], precision='fp32', lambda_group=args.lambda_group)
+        o32 = torch.empty_like(out)
+        p32(o32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            p32(o32)
+        torch.cuda.synchronize()
+        d32 = (time.perf_counter() - t0) / 3
+        ms32 = p32.profile(torch.empty_like(out), repeats=3)
+        extra['batched_fp32'] = {'psfs_per_s': nwave / d32, 'stage2_tflops': p32.info['gemm_flops'][1] / ms32['stage2'] / 1e9,
+                                 'stage_ms': ms32}
+        fp32_img = o32.double().cpu().numpy()
+
+    cpu = None
+    if not args.no_cpu:
+        cps, ref, threads, sample = cpu_baseline(cfg)
+        cpu = {'value': cps, 'unit': 'PSFs/s', 'cores': threads, 'kind': 'port', 'sample': sample}
+        err = float(np.linalg.norm(image - ref) / np.linalg.norm(ref))
+        extra['rel_l2_vs_fp64'] = err
+        if args.precision != 'fp32':
+            extra['batched_fp32']['rel_l2_vs_fp64'] = float(np.linalg.norm(fp32_img - ref) / np.linalg.norm(ref))
+        if not args.no_plain:
+            extra['plain_fp32_per_wavelength']['rel_l2_vs_fp64'] = float(np.linalg.norm(plain_img - ref) / np.linalg.norm(ref))
+        extra['speedup_vs_cpu'] = psfs_per_s / cps
+
+    line = {
+        'metric': 'PSFs/sec (512^2 pupil -> 1024^2 det) + % MFMA peak at <=1e-5 rel-L2 vs FP64',
+        'value': psfs_per_s, 'unit': 'PSFs/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': ms_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': args.precision, 'data': 'synthetic (seeded pupil, Zernike OPD)',
+        'config': {'workload': 'broadband_512', 'pupil': 512, 'detector': 1024, 'wavelengths': nwave,
+                   'psfs_per_step': nwave, 'precision': args.precision, 'lambda_group': args.lambda_group,
+                   'parallelism': 'single GPU'},
+        'tflops_algorithmic': total_flops * args.steps / dt / 1e12,
+        'stage_ms': stage_ms,
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

@@ -34,7 +34,10 @@ def to_dev(x, real=False):
             a = a.astype(np.complex128, copy=False)
         else:
             a = a.astype(np.float64, copy=False)
-        t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        a = np.ascontiguousarray(a)
+        if not a.flags.writeable:
+            a = a.copy()
+        t = torch.from_numpy(a).to(dev)
     if real and t.is_complex():
         raise TypeError('expected a real array')
     if t.dim() == 2 and t.stride(1) != 1:

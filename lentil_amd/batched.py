@@ -194,6 +194,18 @@ class PSFPlan:
         _lib.check(rc, ctx, 'mftx_plan_execute')
         return out
 
+    def profile(self, out=None, repeats=5):
+        """Mean milliseconds of (phasors, stage-1 GEMM, stage-2 GEMM, reduction),
+        timed with HIP events on the launch stream."""
+        if out is None:
+            out = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
+        ms = (ctypes.c_double * 4)()
+        ctx = _lib.context()
+        rc = _lib.load().mftx_plan_profile(self._plan, ctypes.c_void_p(out.data_ptr()), _lib.dtype_code(out),
+                                           out.stride(0), int(repeats), ms)
+        _lib.check(rc, ctx, 'mftx_plan_profile')
+        return dict(phasor=ms[0], stage1=ms[1], stage2=ms[2], reduce=ms[3])
+
     def set_opd(self, tile, opd):
         """Overwrite a tile's OPD in place (the plan re-reads it on every call)."""
         self._amp_opd[tile][1].copy_(torch.as_tensor(opd, dtype=torch.float64))

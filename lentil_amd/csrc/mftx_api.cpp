@@ -836,6 +836,62 @@ int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
   return MFTX_OK;
 }
 
+int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi, int32_t repeats,
+                      double* ms) {
+  if (!plan || !plan->ctx) return MFTX_EHANDLE;
+  mftx_ctx* ctx = plan->ctx;
+  if (!image || !ms || repeats <= 0) return fail(ctx, MFTX_EINVAL, "bad profile request");
+  if (image_dtype != 1 && image_dtype != 3) return fail(ctx, MFTX_EINVAL, "image must be float32 or float64");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  hipEvent_t ev[5];
+  for (auto& e : ev) HIPCHK(ctx, hipEventCreate(&e));
+  double acc[4] = {0, 0, 0, 0};
+  int rc = MFTX_OK;
+  for (int r = 0; r < repeats && rc == MFTX_OK; ++r) {
+    hipError_t e = hipEventRecord(ev[0], s);
+    for (auto& ph : plan->phasors)
+      if (e == hipSuccess)
+        e = launch_phasor(plan->prec, ph.tile.amp, ph.tile.amp_dtype, ph.tile.ld_amp, ph.tile.opd, ph.tile.opd_dtype,
+                          ph.tile.ld_opd, (int)ph.tile.m, (int)ph.tile.n, plan->inv_wl + ph.wave_off, ph.nwave,
+                          ph.dst, ph.wave_stride, ph.rows_pad, ph.kpad, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+    if (e == hipSuccess) e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->works1, plan->nwork1, plan->probs1, plan->segs, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+    if (e == hipSuccess) e = launch_gemm(plan->prec, plan->epi2, plan->works2, plan->nwork2, plan->probs2, plan->segs, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[3], s);
+    if (e != hipSuccess) {
+      rc = hipfail(ctx, e, "profile launch");
+      break;
+    }
+    // the detector reduction (or the coherent fallback) via the regular path
+    // would re-run every stage; time only its own launches
+    if (plan->epi2 == EPI_INTENSITY) {
+      for (auto& red : plan->reduces) {
+        e = launch_reduce_partials(red.partial, red.gstride, red.ngroups, red.rows, red.cols, red.ldp, image,
+                                   image_dtype, ldi, (int)plan->img_rows, (int)plan->img_cols, (int)red.r0,
+                                   (int)red.c0, 0, s);
+        if (e != hipSuccess) break;
+      }
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[4], s);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
+    if (e != hipSuccess) {
+      rc = hipfail(ctx, e, "profile run");
+      break;
+    }
+    for (int k = 0; k < 4; ++k) {
+      float t = 0;
+      hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+      acc[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  for (int k = 0; k < 4; ++k) ms[k] = acc[k] / repeats;
+  return MFTX_OK;
+}
+
 int mftx_plan_set_graph(mftx_plan* plan, int32_t use_graph) {
   if (!plan) return MFTX_EHANDLE;
   plan->use_graph = use_graph ? 1 : 0;
