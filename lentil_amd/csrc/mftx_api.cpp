@@ -1,9 +1,19 @@
 // C-ABI of the mftx library (declared in include/mftx.h).
+//
+// An MFT F = c * E1 . f . E2 (lentil fourier.py:97) runs as two launches of
+// the complex GEMM C = A . B (A packed data, B generated twiddles):
+//   order B (row-first):    G   = f . E2        -> stored as G^T
+//                           F^T = G^T . E1^T    -> F or |F|^2
+//   order A (column-first): T^T = f^T . E1^T    -> stored as T
+//                           F   = T . E2        -> F or |F|^2
+// Order B costs m*n*N + M*m*N complex MACs, order A M*m*n + M*n*N; the
+// cheaper one is used.  Several Fields summed coherently share one output
+// grid, so their stage-2 GEMMs concatenate along K (one GEMM per coherent
+// group, each K segment with its own twiddle coordinate origin).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
-#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -18,10 +28,50 @@ using namespace mftx;
 namespace {
 
 constexpr int kRingSlots = 16;
-constexpr size_t kSlotBytes = 512 * 1024;   // up to ~13k work items per launch
+constexpr size_t kSlotBytes = 512 * 1024;  // up to ~13k work items per launch
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
-int64_t floor_half(int64_t k) { return k / 2; }  // floor(k/2) for k >= 0
+double centre(int64_t k) { return (double)(k / 2); }  // floor(k/2), k >= 0 (fourier.py:117-120)
+
+// MACs of the two contraction orders for an m x n -> M x N transform
+double cost_row_first(double m, double n, double M, double N) { return m * n * N + M * m * N; }
+double cost_col_first(double m, double n, double M, double N) { return M * m * n + M * n * N; }
+
+Work empty_work() { return Work{0, 0, 0, 0, nullptr, 0, 0, 0}; }
+
+// XCD-aware order: clusters of works that share A rows are assigned round-
+// robin to the 8 XCDs; workgroup i of a launch runs on XCD i % 8, so a
+// cluster's works land on one XCD and share its L2.
+std::vector<Work> xcd_order(const std::vector<std::vector<Work>>& clusters) {
+  std::vector<std::vector<Work>> bins(8);
+  for (size_t c = 0; c < clusters.size(); ++c) {
+    auto& b = bins[c % 8];
+    b.insert(b.end(), clusters[c].begin(), clusters[c].end());
+  }
+  size_t mx = 0;
+  for (auto& b : bins) mx = std::max(mx, b.size());
+  std::vector<Work> out;
+  out.reserve(mx * 8);
+  for (size_t i = 0; i < mx; ++i)
+    for (int x = 0; x < 8; ++x) out.push_back(i < bins[x].size() ? bins[x][i] : empty_work());
+  while (!out.empty() && out.back().pend == 0) out.pop_back();
+  return out;
+}
+
+// Works covering a C of rows_tiled x cols for problems [pb, pe), one cluster per tile row.
+void add_tiles(std::vector<std::vector<Work>>& clusters, int pb, int pe, int64_t rows_tiled, int64_t cols,
+               float* partial = nullptr, long long ldp = 0, int prows = 0, int pcols = 0) {
+  const int tr = (int)(rows_tiled / BM), tc = (int)((cols + BN - 1) / BN);
+  for (int r = 0; r < tr; ++r) {
+    std::vector<Work> row;
+    for (int c = 0; c < tc; ++c) row.push_back(Work{pb, pe, r, c, partial, ldp, prows, pcols});
+    clusters.push_back(row);
+  }
+}
+
+void push_chunks(std::vector<double>& cx, int kpad, double x0) {
+  for (int c = 0; c < kpad / BK; ++c) cx.push_back(x0 + (double)(c * BK));
+}
 
 }  // namespace
 
@@ -43,46 +93,45 @@ struct mftx_plan {
   mftx_ctx* ctx = nullptr;
   int prec = 0;
   int64_t img_rows = 0, img_cols = 0;
-  // device workspace and descriptor arrays
-  void* ws = nullptr;
-  size_t ws_bytes = 0;
+  void* ws = nullptr;  // workspace: operands, intermediates, partial images, descriptors
   Work* works1 = nullptr;
   Work* works2 = nullptr;
   Problem* probs1 = nullptr;
   Problem* probs2 = nullptr;
-  Segment* segs = nullptr;
+  double* chunk_x = nullptr;
   double* inv_wl = nullptr;
   int nwork1 = 0, nwork2 = 0;
   int epi2 = EPI_INTENSITY;
-  // phasor launches
   struct Phasor {
     mftx_tile tile;
-    int nwave;
-    int wave_off;          // offset into inv_wl
+    int trans, nwave, wave_off;
     void* dst;
-    long long wave_stride; // bytes
+    long long wave_stride;  // bytes
     long long rows_pad;
     int kpad;
   };
   std::vector<Phasor> phasors;
-  // reductions of partial images onto the detector
   struct Reduce {
     float* partial;
-    long long gstride;
+    long long gstride, ldp;
     int ngroups, rows, cols;
-    long long ldp;
     long long r0, c0;
   };
   std::vector<Reduce> reduces;
-  // fallback (partially overlapping coherent groups): complex accumulators
+  // coherent groups whose Fields overlap only partially: complex outputs
+  // summed into an image-sized accumulator, then |.|^2 (field.py:331-346)
+  struct Add {
+    const float2* F;
+    int M, N;
+    int64_t r0, c0;
+  };
   struct Fallback {
-    float2* acc;                    // image-sized complex accumulator for the group
-    std::vector<std::tuple<const float2*, int, int, int64_t, int64_t, long long>> adds;  // F, M, N, r0, c0, ldF
+    float2* acc;
+    std::vector<Add> adds;
     double weight;
   };
   std::vector<Fallback> fallbacks;
   mftx_plan_info info = {};
-  // graph replay
   int use_graph = 0;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -100,10 +149,10 @@ static int hipfail(mftx_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, MFTX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIPCHK(ctx, expr)                                      \
-  do {                                                         \
-    hipError_t e_ = (expr);                                    \
-    if (e_ != hipSuccess) return hipfail((ctx), e_, #expr);    \
+#define HIPCHK(ctx, expr)                                   \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return hipfail((ctx), e_, #expr); \
   } while (0)
 
 static int ensure_ws(mftx_ctx* ctx, size_t bytes) {
@@ -114,7 +163,7 @@ static int ensure_ws(mftx_ctx* ctx, size_t bytes) {
     ctx->ws = nullptr;
     ctx->ws_bytes = 0;
   }
-  size_t nb = std::max(bytes, (size_t)(ctx->ws_bytes * 3 / 2));
+  const size_t nb = std::max(bytes, (size_t)(ctx->ws_bytes * 3 / 2));
   if (hipMalloc(&ctx->ws, nb) != hipSuccess) {
     ctx->ws = nullptr;
     return fail(ctx, MFTX_ENOMEM, "workspace allocation of " + std::to_string(nb) + " bytes failed");
@@ -126,7 +175,7 @@ static int ensure_ws(mftx_ctx* ctx, size_t bytes) {
 // Stage `bytes` of descriptors into a ring slot and queue the upload on the
 // context stream; returns the device address of the slot.
 static int upload_desc(mftx_ctx* ctx, const void* src, size_t bytes, void** dev) {
-  if (bytes > kSlotBytes) return fail(ctx, MFTX_EINVAL, "descriptor block too large");
+  if (bytes > kSlotBytes) return fail(ctx, MFTX_EINVAL, "descriptor block too large (output too big for one call)");
   const int s = ctx->ring_next;
   ctx->ring_next = (s + 1) % kRingSlots;
   if (ctx->ring_used[s]) HIPCHK(ctx, hipEventSynchronize(ctx->ring_ev[s]));
@@ -140,9 +189,32 @@ static int upload_desc(mftx_ctx* ctx, const void* src, size_t bytes, void** dev)
   return MFTX_OK;
 }
 
+// One GEMM launch of a single problem covering rows_tiled x p.cols of C.
+static int run_single(mftx_ctx* ctx, int prec, int epi, Problem p, double x0, int64_t rows_tiled) {
+  std::vector<std::vector<Work>> clusters;
+  add_tiles(clusters, 0, 1, rows_tiled, p.cols);
+  const std::vector<Work> works = xcd_order(clusters);
+  std::vector<double> cx;
+  push_chunks(cx, p.kpad, x0);
+  p.chunk_begin = 0;
+  const size_t off_cx = round_up(sizeof(Problem), 16);
+  const size_t off_w = round_up(off_cx + sizeof(double) * cx.size(), 16);
+  std::vector<char> host(off_w + sizeof(Work) * works.size());
+  std::memcpy(host.data(), &p, sizeof(Problem));
+  std::memcpy(host.data() + off_cx, cx.data(), sizeof(double) * cx.size());
+  std::memcpy(host.data() + off_w, works.data(), sizeof(Work) * works.size());
+  void* dev = nullptr;
+  int rc = upload_desc(ctx, host.data(), host.size(), &dev);
+  if (rc) return rc;
+  char* d = (char*)dev;
+  HIPCHK(ctx, launch_gemm(prec, epi, (const Work*)(d + off_w), (int)works.size(), (const Problem*)d,
+                          (const double*)(d + off_cx), ctx->stream));
+  return MFTX_OK;
+}
+
 extern "C" {
 
-int mftx_version(void) { return 100; }
+int mftx_version(void) { return 10100; }
 
 int mftx_create(int device, mftx_ctx** out) {
   if (!out) return MFTX_EINVAL;
@@ -155,12 +227,12 @@ int mftx_create(int device, mftx_ctx** out) {
   if (hipSetDevice(device) != hipSuccess ||
       hipHostMalloc((void**)&ctx->ring_host, kRingSlots * kSlotBytes, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&ctx->ring_dev, kRingSlots * kSlotBytes) != hipSuccess) {
-    delete ctx;
+    mftx_destroy(ctx);
     return MFTX_ENOMEM;
   }
   for (int i = 0; i < kRingSlots; ++i) {
     if (hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming) != hipSuccess) {
-      delete ctx;
+      mftx_destroy(ctx);
       return MFTX_EDEVICE;
     }
   }
@@ -196,54 +268,6 @@ int mftx_synchronize(mftx_ctx* ctx) {
   return MFTX_OK;
 }
 
-static bool valid_dtype(int d) { return d >= 0 && d <= 3; }
-
-// Fill a single-problem GEMM descriptor set and launch it on tiles covering
-// rows_pad x cols of C.
-static int run_single(mftx_ctx* ctx, int prec, int epi, const Problem& p, const Segment& sg,
-                      long long rows_tiled) {
-  const int tr = (int)(rows_tiled / BM);
-  const int tc = (int)((p.cols + BN - 1) / BN);
-  const int nw = tr * tc;
-  const size_t need = sizeof(Problem) + sizeof(Segment) + sizeof(Work) * (size_t)nw;
-  std::vector<char> host(need);
-  Problem* hp = (Problem*)host.data();
-  Segment* hs = (Segment*)(host.data() + sizeof(Problem));
-  Work* hw = (Work*)(host.data() + sizeof(Problem) + sizeof(Segment));
-  *hp = p;
-  hp->seg_begin = 0;
-  hp->seg_count = 1;
-  *hs = sg;
-  // XCD-aware order: works of one tile row (sharing A) land on one XCD
-  std::vector<std::vector<Work>> bins(8);
-  for (int r = 0; r < tr; ++r)
-    for (int c = 0; c < tc; ++c) bins[r % 8].push_back(Work{0, 1, r, c, nullptr, 0, 0, 0});
-  size_t mx = 0;
-  for (auto& b : bins) mx = std::max(mx, b.size());
-  std::vector<Work> order;
-  for (size_t i = 0; i < mx; ++i)
-    for (int x = 0; x < 8; ++x) order.push_back(i < bins[x].size() ? bins[x][i] : Work{0, 0, 0, 0, nullptr, 0, 0, 0});
-  // trim trailing empty works
-  while (!order.empty() && order.back().pend == 0) order.pop_back();
-  if (order.size() * sizeof(Work) + sizeof(Problem) + sizeof(Segment) > kSlotBytes) {
-    order.clear();
-    for (int r = 0; r < tr; ++r)
-      for (int c = 0; c < tc; ++c) order.push_back(Work{0, 1, r, c, nullptr, 0, 0, 0});
-  }
-  host.resize(sizeof(Problem) + sizeof(Segment) + sizeof(Work) * order.size());
-  hp = (Problem*)host.data();
-  hw = (Work*)(host.data() + sizeof(Problem) + sizeof(Segment));
-  std::memcpy(hw, order.data(), sizeof(Work) * order.size());
-  void* dev = nullptr;
-  int rc = upload_desc(ctx, host.data(), host.size(), &dev);
-  if (rc) return rc;
-  const Problem* dp = (const Problem*)dev;
-  const Segment* ds = (const Segment*)((char*)dev + sizeof(Problem));
-  const Work* dw = (const Work*)((char*)dev + sizeof(Problem) + sizeof(Segment));
-  HIPCHK(ctx, launch_gemm(prec, epi, dw, (int)order.size(), dp, ds, ctx->stream));
-  return MFTX_OK;
-}
-
 int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, int64_t ldf, void* F, int64_t M,
               int64_t N, int64_t ldF, double alpha_r, double alpha_c, double shift_r, double shift_c,
               double offset_r, double offset_c, double scale, int conj, int precision) {
@@ -251,57 +275,61 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   if (!f || !F) return fail(ctx, MFTX_EINVAL, "null array pointer");
   if (m <= 0 || n <= 0 || M <= 0 || N <= 0) return fail(ctx, MFTX_EINVAL, "shapes must be positive");
   if (ldf < n || ldF < N) return fail(ctx, MFTX_EINVAL, "row stride smaller than row length");
-  if (!valid_dtype(f_dtype)) return fail(ctx, MFTX_EINVAL, "unknown input dtype");
-  if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3)
-    return fail(ctx, MFTX_EINVAL, "unknown precision");
+  if (f_dtype < 0 || f_dtype > 3) return fail(ctx, MFTX_EINVAL, "unknown input dtype");
+  if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3) return fail(ctx, MFTX_EINVAL, "unknown precision");
   if (m > (1 << 20) || n > (1 << 20) || M > (1 << 20) || N > (1 << 20))
     return fail(ctx, MFTX_EINVAL, "dimension too large");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const double sgn = conj ? 1.0 : -1.0;
   const int prec = precision;
-  // order B: G = f . E2 (stage 1, stored as G^T), F^T = G^T . E1^T (stage 2)
-  const int64_t rows1 = round_up(m, BM);
-  const int kpad1 = (int)round_up(n, BK);
-  const int64_t rows2 = round_up(N, BM);
-  const int kpad2 = (int)round_up(m, BK);
+  const bool col_first = cost_col_first(m, n, M, N) < cost_row_first(m, n, M, N);
+  // stage-1 operand: f (row-first) or f^T (column-first)
+  const int64_t r1 = col_first ? n : m, k1 = col_first ? m : n;  // rows and K of stage 1
+  const int64_t c1 = col_first ? M : N;                           // stage-1 output columns
+  const int64_t c2 = col_first ? N : M;                           // stage-2 output columns
+  const int64_t rows1 = round_up(r1, BM);
+  const int kpad1 = (int)round_up(k1, BK);
+  const int64_t rows2 = round_up(c1, BM);
+  const int kpad2 = (int)round_up(r1, BK);
   const size_t b1 = packed_bytes(prec, rows1, kpad1);
   const size_t b2 = packed_bytes(prec, rows2, kpad2);
-  int rc = ensure_ws(ctx, b1 + b2 + 256);
+  int rc = ensure_ws(ctx, (size_t)round_up((int64_t)b1, 256) + b2);
   if (rc) return rc;
   char* a1 = (char*)ctx->ws;
   char* a2 = a1 + round_up((int64_t)b1, 256);
-  HIPCHK(ctx, launch_pack(prec, f, f_dtype, m, n, ldf, a1, rows1, kpad1, ctx->stream));
+  HIPCHK(ctx, launch_pack(prec, f, f_dtype, m, n, ldf, col_first ? 1 : 0, a1, rows1, kpad1, ctx->stream));
 
+  // twiddle coordinates: rows x = R + or, U - sr;  columns x = S + oc, V - sc
+  const double xr0 = -centre(m) + offset_r, yu0 = -centre(M) - shift_r;
+  const double xc0 = -centre(n) + offset_c, yv0 = -centre(N) - shift_c;
   Problem p1 = {};
   p1.a = a1;
   p1.lda = packed_lda(prec, kpad1);
-  p1.alpha = sgn * alpha_c;
-  p1.y0 = -(double)floor_half(N) - shift_c;
-  p1.rows = (int)m;
-  p1.cols = (int)N;
+  p1.alpha = sgn * (col_first ? alpha_r : alpha_c);
+  p1.y0 = col_first ? yu0 : yv0;
+  p1.rows = (int)r1;
+  p1.cols = (int)c1;
   p1.kpad = kpad1;
   p1.scale = 1.0f;
   p1.limit = kpad2;
   p1.dst = a2;
   p1.ldd = packed_lda(prec, kpad2);
-  Segment s1 = {0, kpad1, -(double)floor_half(n) + offset_c};
-  rc = run_single(ctx, prec, EPI_STORE_PACKED_T, p1, s1, rows1);
+  rc = run_single(ctx, prec, EPI_STORE_PACKED_T, p1, col_first ? xr0 : xc0, rows1);
   if (rc) return rc;
 
   Problem p2 = {};
   p2.a = a2;
   p2.lda = packed_lda(prec, kpad2);
-  p2.alpha = sgn * alpha_r;
-  p2.y0 = -(double)floor_half(M) - shift_r;
-  p2.rows = (int)N;
-  p2.cols = (int)M;
+  p2.alpha = sgn * (col_first ? alpha_c : alpha_r);
+  p2.y0 = col_first ? yv0 : yu0;
+  p2.rows = (int)c1;
+  p2.cols = (int)c2;
   p2.kpad = kpad2;
   p2.scale = (float)scale;
   p2.dst = F;
   p2.ldd = ldF;
-  p2.trans = 1;
-  Segment s2 = {0, kpad2, -(double)floor_half(m) + offset_r};
-  return run_single(ctx, prec, EPI_STORE_CPLX, p2, s2, rows2);
+  p2.trans = col_first ? 0 : 1;  // C is F (column-first) or F^T (row-first)
+  return run_single(ctx, prec, EPI_STORE_CPLX, p2, col_first ? xc0 : xr0, rows2);
 }
 
 int mftx_phasor(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd, int opd_dtype,
@@ -349,14 +377,16 @@ int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes) {
   return MFTX_OK;
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------------------
 // batched plan
 // ---------------------------------------------------------------------------
 
-}  // extern "C"
-
 namespace {
 
+// Problems with equal keys have one output grid and one stage-1 twiddle set
+// (per member tile), so their stage-2 GEMMs concatenate along K.
 struct ClassKey {
   int wave, M, N;
   double ar, ac, sr, sc;
@@ -366,23 +396,27 @@ struct ClassKey {
   }
 };
 
-// XCD-aware interleave: clusters of works that share the same A rows are
-// assigned round-robin to the 8 XCDs; workgroup i runs on XCD i % 8.
-std::vector<Work> xcd_order(const std::vector<std::vector<Work>>& clusters) {
-  std::vector<std::vector<Work>> bins(8);
-  for (size_t c = 0; c < clusters.size(); ++c) {
-    auto& b = bins[c % 8];
-    b.insert(b.end(), clusters[c].begin(), clusters[c].end());
+struct Layout {
+  size_t total = 256;  // leading pad: no real buffer sits at offset 0
+  size_t take(size_t bytes) {
+    const size_t off = total;
+    total += (size_t)round_up((int64_t)bytes, 256);
+    return off;
   }
-  size_t mx = 0;
-  for (auto& b : bins) mx = std::max(mx, b.size());
-  std::vector<Work> out;
-  out.reserve(mx * 8);
-  for (size_t i = 0; i < mx; ++i)
-    for (int x = 0; x < 8; ++x) out.push_back(i < bins[x].size() ? bins[x][i] : Work{0, 0, 0, 0, nullptr, 0, 0, 0});
-  while (!out.empty() && out.back().pend == 0) out.pop_back();
-  return out;
-}
+};
+
+struct ClassInfo {
+  ClassKey key;
+  std::vector<int> members;  // problem indices
+  int group;
+  bool col_first;
+  int64_t rows2;      // stage-2 A rows (tile-padded): N (row-first) or M (column-first)
+  int ktot;           // stage-2 K
+  std::vector<int> kbeg;
+  size_t abuf = 0;    // stage-2 A buffer
+  size_t fbuf = 0;    // complex F (partial-overlap groups)
+  int p2 = -1;        // index of its stage-2 problem
+};
 
 }  // namespace
 
@@ -400,8 +434,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   if (lambda_group <= 0) lambda_group = 4;
   for (int t = 0; t < ntiles; ++t) {
     const mftx_tile& T = tiles[t];
-    if (!T.amp || !T.opd || T.m <= 0 || T.n <= 0 || T.ld_amp < T.n || T.ld_opd < T.n ||
-        (T.amp_dtype != 1 && T.amp_dtype != 3) || (T.opd_dtype != 1 && T.opd_dtype != 3))
+    if (!T.amp || !T.opd || T.m <= 0 || T.n <= 0 || T.m > (1 << 20) || T.n > (1 << 20) || T.ld_amp < T.n ||
+        T.ld_opd < T.n || (T.amp_dtype != 1 && T.amp_dtype != 3) || (T.opd_dtype != 1 && T.opd_dtype != 3))
       return fail(ctx, MFTX_EINVAL, "bad tile " + std::to_string(t));
   }
   for (int w = 0; w < nwave; ++w)
@@ -414,329 +448,306 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int prec = precision;
+
+  // ---- classes and coherent groups ---------------------------------------------
+  std::map<ClassKey, int> class_of;
+  std::vector<ClassInfo> cls;
+  for (int i = 0; i < nprob; ++i) {
+    const mftx_problem& P = problems[i];
+    const ClassKey k{P.wave, P.M, P.N, P.alpha_r, P.alpha_c, P.shift_r, P.shift_c, P.out_r0, P.out_c0};
+    auto it = class_of.find(k);
+    if (it == class_of.end()) {
+      it = class_of.emplace(k, (int)cls.size()).first;
+      ClassInfo c;
+      c.key = k;
+      c.group = P.group;
+      cls.push_back(c);
+    }
+    if (cls[it->second].group != P.group)
+      return fail(ctx, MFTX_EINVAL, "problems with identical output geometry must share a coherent group");
+    cls[it->second].members.push_back(i);
+  }
+  std::map<std::pair<int, int>, std::vector<int>> groups;  // (wave, group) -> classes
+  for (size_t c = 0; c < cls.size(); ++c) groups[{cls[c].key.wave, cls[c].group}].push_back((int)c);
+  bool fast = true;
+  for (auto& g : groups)
+    if (g.second.size() > 1) fast = false;
+
+  // contraction order per class (all members share M, N)
+  for (auto& c : cls) {
+    double row_first = 0, col_first = 0;
+    for (int i : c.members) {
+      const mftx_tile& T = tiles[problems[i].tile];
+      row_first += cost_row_first(T.m, T.n, c.key.M, c.key.N);
+      col_first += cost_col_first(T.m, T.n, c.key.M, c.key.N);
+    }
+    c.col_first = col_first < row_first;
+  }
+
+  // ---- workspace layout ----------------------------------------------------------
+  Layout lay;
+  // phasor operands per (tile, orientation): every wavelength the pair is used with
+  std::map<std::pair<int, int>, std::vector<int>> tile_waves;
+  for (auto& c : cls)
+    for (int i : c.members) tile_waves[{problems[i].tile, c.col_first ? 1 : 0}].push_back(problems[i].wave);
+  struct TileOp { int64_t rows; int kpad; size_t off; std::map<int, int> slot; int wave_off; std::vector<int> waves; };
+  std::map<std::pair<int, int>, TileOp> tops;
+  std::vector<int> inv_order;
+  for (auto& kv : tile_waves) {
+    auto v = kv.second;
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    const mftx_tile& T = tiles[kv.first.first];
+    const bool tr = kv.first.second == 1;
+    TileOp op;
+    op.rows = round_up(tr ? T.n : T.m, BM);
+    op.kpad = (int)round_up(tr ? T.m : T.n, BK);
+    op.wave_off = (int)inv_order.size();
+    op.waves = v;
+    for (size_t j = 0; j < v.size(); ++j) {
+      op.slot[v[j]] = (int)j;
+      inv_order.push_back(v[j]);
+    }
+    op.off = lay.take(packed_bytes(prec, op.rows, op.kpad) * v.size());
+    tops[kv.first] = op;
+  }
+  // stage-2 operands per class
+  for (auto& c : cls) {
+    c.rows2 = round_up(c.col_first ? c.key.M : c.key.N, BM);
+    int k = 0;
+    for (int i : c.members) {
+      const mftx_tile& T = tiles[problems[i].tile];
+      c.kbeg.push_back(k);
+      k += (int)round_up(c.col_first ? T.n : T.m, BK);
+    }
+    c.ktot = k;
+    c.abuf = lay.take(packed_bytes(prec, c.rows2, c.ktot));
+    if (!fast) c.fbuf = lay.take(sizeof(float2) * (size_t)c.key.M * c.key.N);
+  }
+
+  // ---- descriptors ---------------------------------------------------------------
+  std::vector<Problem> p1, p2;
+  std::vector<double> cx;
+  std::vector<std::vector<Work>> clus1, clus2;
+  struct Patch { int idx; size_t a, dst; };  // workspace offsets to rebase after allocation
+  std::vector<Patch> patch1, patch2;
+
+  for (auto& c : cls) {
+    const ClassKey& K = c.key;
+    const bool cf = c.col_first;
+    // stage 1, one problem per member
+    for (size_t j = 0; j < c.members.size(); ++j) {
+      const mftx_problem& P = problems[c.members[j]];
+      const mftx_tile& T = tiles[P.tile];
+      const TileOp& op = tops[{P.tile, cf ? 1 : 0}];
+      const int kp2 = (int)round_up(cf ? T.n : T.m, BK);
+      Problem q = {};
+      q.lda = packed_lda(prec, op.kpad);
+      q.alpha = -(cf ? P.alpha_r : P.alpha_c);
+      q.y0 = cf ? -centre(K.M) - K.sr : -centre(K.N) - K.sc;
+      q.rows = (int)(cf ? T.n : T.m);
+      q.cols = cf ? K.M : K.N;
+      q.kpad = op.kpad;
+      q.chunk_begin = (int)cx.size();
+      push_chunks(cx, op.kpad, cf ? -centre(T.m) + P.offset_r : -centre(T.n) + P.offset_c);
+      q.scale = 1.0f;
+      q.limit = kp2;
+      q.ldd = packed_lda(prec, c.ktot);
+      const size_t elem = prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(uint16_t);
+      patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)op.slot.at(P.wave) * packed_bytes(prec, op.rows, op.kpad),
+                             c.abuf + (size_t)c.kbeg[j] * elem});
+      add_tiles(clus1, (int)p1.size(), (int)p1.size() + 1, op.rows, q.cols);
+      p1.push_back(q);
+    }
+    // stage 2, one problem per class (K concatenated over members)
+    Problem q = {};
+    q.lda = packed_lda(prec, c.ktot);
+    q.alpha = -(cf ? K.ac : K.ar);
+    q.y0 = cf ? -centre(K.N) - K.sc : -centre(K.M) - K.sr;
+    q.rows = cf ? K.M : K.N;
+    q.cols = cf ? K.N : K.M;
+    q.kpad = c.ktot;
+    q.chunk_begin = (int)cx.size();
+    for (size_t j = 0; j < c.members.size(); ++j) {
+      const mftx_problem& P = problems[c.members[j]];
+      const mftx_tile& T = tiles[P.tile];
+      push_chunks(cx, (int)round_up(cf ? T.n : T.m, BK), cf ? -centre(T.n) + P.offset_c : -centre(T.m) + P.offset_r);
+    }
+    q.trans = cf ? 0 : 1;
+    const double unit = std::sqrt(std::fabs(K.ar * K.ac));  // unitary scale (fourier.py:100-101)
+    if (fast) {
+      q.scale = (float)(unit * unit * problems[c.members[0]].weight);
+    } else {
+      q.scale = (float)unit;
+      q.ldd = K.N;  // F[u][v]
+    }
+    c.p2 = (int)p2.size();
+    patch2.push_back(Patch{(int)p2.size(), c.abuf, fast ? 0 : c.fbuf});
+    p2.push_back(q);
+  }
+
   mftx_plan* plan = new mftx_plan();
   plan->ctx = ctx;
   plan->prec = prec;
   plan->img_rows = image_rows;
   plan->img_cols = image_cols;
 
-  // ---- classes: problems that can share one K-concatenated stage-2 GEMM ----
-  std::map<ClassKey, std::vector<int>> classes;
-  for (int i = 0; i < nprob; ++i) {
-    const mftx_problem& P = problems[i];
-    classes[ClassKey{P.wave, P.M, P.N, P.alpha_r, P.alpha_c, P.shift_r, P.shift_c, P.out_r0, P.out_c0}].push_back(i);
-  }
-  // coherent groups -> classes
-  std::map<std::pair<int, int>, std::vector<const std::vector<int>*>> groups;  // (wave, group) -> classes
-  for (auto& kv : classes) {
-    // a class must not mix coherent groups; split it if it does
-    groups[{kv.first.wave, problems[kv.second[0]].group}].push_back(&kv.second);
-  }
-  for (auto& kv : classes)
-    for (int i : kv.second)
-      if (problems[i].group != problems[kv.second[0]].group) {
-        delete plan;
-        return fail(ctx, MFTX_EINVAL, "problems with identical geometry must share a coherent group");
-      }
-  bool fast = true;
-  for (auto& g : groups)
-    if (g.second.size() > 1) fast = false;
-  plan->info.coherent_fast = fast ? 1 : 0;
-
-  // ---- workspace layout ----------------------------------------------------
-  struct Buf { size_t off, bytes; };
-  size_t ws_total = 0;
-  auto reserve = [&](size_t bytes) { Buf b{ws_total, bytes}; ws_total += (size_t)round_up((int64_t)bytes, 256); return b; };
-  // Buffer offsets are carried in pointer fields until the workspace exists;
-  // a leading pad keeps every real offset non-zero.
-  reserve(256);
-
-  // phasor buffers: per tile, all wavelengths the tile is used with
-  std::vector<std::vector<int>> tile_waves(ntiles);
-  for (int i = 0; i < nprob; ++i) tile_waves[problems[i].tile].push_back(problems[i].wave);
-  std::vector<int> inv_order;        // wavelength index list for inv_wl upload
-  std::vector<std::map<int, int>> tile_wave_slot(ntiles);
-  std::vector<Buf> tile_buf(ntiles);
-  std::vector<int64_t> tile_rows(ntiles), tile_kpad(ntiles);
-  std::vector<int> tile_wave_off(ntiles);
-  for (int t = 0; t < ntiles; ++t) {
-    auto& v = tile_waves[t];
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
-    tile_rows[t] = round_up(tiles[t].m, BM);
-    tile_kpad[t] = round_up(tiles[t].n, BK);
-    tile_wave_off[t] = (int)inv_order.size();
-    for (size_t j = 0; j < v.size(); ++j) {
-      tile_wave_slot[t][v[j]] = (int)j;
-      inv_order.push_back(v[j]);
-    }
-    tile_buf[t] = reserve(packed_bytes(prec, tile_rows[t], (int)tile_kpad[t]) * v.size());
-  }
-
-  // stage-2 A buffers: one per class, K = sum of member kpads (roundup(m_t, BK))
-  struct ClassInfo {
-    const std::vector<int>* members;
-    ClassKey key;
-    int64_t rows2;   // round_up(N, BM)
-    int ktot;
-    Buf abuf;
-    Buf fbuf;        // fallback complex output
-    std::vector<int> kbeg;
-  };
-  std::vector<ClassInfo> cinfo;
-  for (auto& kv : classes) {
-    ClassInfo ci;
-    ci.members = &kv.second;
-    ci.key = kv.first;
-    ci.rows2 = round_up(kv.first.N, BM);
-    int k = 0;
-    for (int i : kv.second) {
-      ci.kbeg.push_back(k);
-      k += (int)round_up(tiles[problems[i].tile].m, BK);
-    }
-    ci.ktot = k;
-    ci.abuf = reserve(packed_bytes(prec, ci.rows2, ci.ktot));
-    if (!fast) ci.fbuf = reserve((size_t)kv.first.M * kv.first.N * sizeof(float2));
-    cinfo.push_back(ci);
-  }
-
-  // λ-groups for the intensity epilogue: classes with identical output
-  // geometry (M, N, r0, c0) in chunks of lambda_group
-  std::map<std::tuple<int, int, int64_t, int64_t>, std::vector<int>> geo;  // -> class indices
-  if (fast)
-    for (size_t c = 0; c < cinfo.size(); ++c)
-      geo[{cinfo[c].key.M, cinfo[c].key.N, cinfo[c].key.r0, cinfo[c].key.c0}].push_back((int)c);
-  struct GeoInfo { std::vector<std::vector<int>> lgroups; Buf partial; int64_t prow, pcol; };
-  std::vector<GeoInfo> ginfo;
-  for (auto& kv : geo) {
-    GeoInfo gi;
-    auto& cl = kv.second;
-    for (size_t s = 0; s < cl.size(); s += lambda_group)
-      gi.lgroups.emplace_back(cl.begin() + s, cl.begin() + std::min(cl.size(), s + (size_t)lambda_group));
-    gi.prow = round_up(std::get<0>(kv.first), BN);  // partial[u][v]: u over M (C cols)
-    gi.pcol = round_up(std::get<1>(kv.first), BM);  // v over N (C rows)
-    gi.partial = reserve(sizeof(float) * gi.prow * gi.pcol * gi.lgroups.size());
-    ginfo.push_back(gi);
-  }
-  std::vector<Buf> fb_acc;
-  std::map<std::pair<int, int>, int> group_fb;
-  if (!fast) {
-    for (auto& g : groups) {
-      group_fb[g.first] = (int)fb_acc.size();
-      fb_acc.push_back(reserve(sizeof(float2) * image_rows * image_cols));
-    }
-  }
-
-  // descriptor arrays (sizes known after building them on the host)
-  std::vector<Problem> p1, p2;
-  std::vector<Segment> segs;
-  std::vector<std::vector<Work>> clus1, clus2;
-
-  // stage 1: one problem per (tile, λ) member, output into its class A buffer
-  // stage 2: one problem per class
-  for (size_t c = 0; c < cinfo.size(); ++c) {
-    ClassInfo& ci = cinfo[c];
-    const ClassKey& K = ci.key;
-    Problem q2 = {};
-    q2.lda = packed_lda(prec, ci.ktot);
-    q2.alpha = -K.ar;
-    q2.y0 = -(double)floor_half(K.M) - K.sr;
-    q2.rows = K.N;
-    q2.cols = K.M;
-    q2.kpad = ci.ktot;
-    q2.seg_begin = (int)segs.size();
-    q2.seg_count = (int)ci.members->size();
-    q2.trans = 1;
-    for (size_t j = 0; j < ci.members->size(); ++j) {
-      const mftx_problem& P = problems[(*ci.members)[j]];
-      const mftx_tile& T = tiles[P.tile];
-      const int kp2 = (int)round_up(T.m, BK);
-      segs.push_back(Segment{ci.kbeg[j], ci.kbeg[j] + kp2, -(double)floor_half(T.m) + P.offset_r});
-      Problem q1 = {};
-      const int slot = tile_wave_slot[P.tile][P.wave];
-      q1.a = (void*)(tile_buf[P.tile].off + (size_t)slot * packed_bytes(prec, tile_rows[P.tile], (int)tile_kpad[P.tile]));
-      q1.lda = packed_lda(prec, (int)tile_kpad[P.tile]);
-      q1.alpha = -P.alpha_c;
-      q1.y0 = -(double)floor_half(P.N) - P.shift_c;
-      q1.rows = (int)T.m;
-      q1.cols = P.N;
-      q1.kpad = (int)tile_kpad[P.tile];
-      q1.seg_begin = -1;  // patched below (own segment)
-      q1.seg_count = 1;
-      q1.scale = 1.0f;
-      q1.limit = kp2;
-      q1.dst = (void*)(ci.abuf.off + (size_t)ci.kbeg[j] * (prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(__bf16)));
-      q1.ldd = packed_lda(prec, ci.ktot);
-      const int pi1 = (int)p1.size();
-      p1.push_back(q1);
-      // stage-1 segment
-      // (appended after all class segments; index fixed up below)
-      std::vector<Work> cl;
-      const int trs = (int)(tile_rows[P.tile] / BM), tcs = (int)((P.N + BN - 1) / BN);
-      for (int r = 0; r < trs; ++r) {
-        std::vector<Work> row;
-        for (int cc = 0; cc < tcs; ++cc) row.push_back(Work{pi1, pi1 + 1, r, cc, nullptr, 0, 0, 0});
-        clus1.push_back(row);
-      }
-    }
-    const double unit = std::sqrt(std::fabs(K.ar * K.ac));
-    q2.scale = (float)(unit * unit * problems[(*ci.members)[0]].weight);
-    if (!fast) {
-      q2.scale = (float)unit;
-      q2.dst = (void*)ci.fbuf.off;
-      q2.ldd = K.N;  // F[u][v], u < M
-    }
-    p2.push_back(q2);
-  }
-  // stage-1 segments: x_s = s - floor(n/2) + offset_c
-  for (size_t i = 0, c = 0; c < cinfo.size(); ++c)
-    for (size_t j = 0; j < cinfo[c].members->size(); ++j, ++i) {
-      const mftx_problem& P = problems[(*cinfo[c].members)[j]];
-      const mftx_tile& T = tiles[P.tile];
-      p1[i].seg_begin = (int)segs.size();
-      segs.push_back(Segment{0, (int)tile_kpad[P.tile], -(double)floor_half(T.n) + P.offset_c});
-    }
-  // stage-2 works
+  // stage-2 works: λ-groups of classes with one output geometry and orientation
+  // accumulate |F|^2 in registers (fast path); otherwise one work set per class
+  struct GeoInfo { std::vector<std::vector<int>> lgroups; size_t partial; int64_t prow, pcol; };
+  std::vector<GeoInfo> geos;
   if (fast) {
-    for (auto& gi : ginfo) {
-      for (size_t g = 0; g < gi.lgroups.size(); ++g) {
-        const auto& lg = gi.lgroups[g];
-        const ClassInfo& c0 = cinfo[lg[0]];
-        // problems of a λ-group must be contiguous in p2: they are (class order)
-        // only if lg is a contiguous run; build explicit contiguous copies
-        const int pb = (int)p2.size();
-        for (int c : lg) p2.push_back(p2[c]);
-        const int pe = (int)p2.size();
-        const int trs = (int)(c0.rows2 / BM), tcs = (int)((c0.key.M + BN - 1) / BN);
-        for (int r = 0; r < trs; ++r) {
-          std::vector<Work> row;
-          for (int cc = 0; cc < tcs; ++cc)
-            row.push_back(Work{pb, pe, r, cc, (float*)(gi.partial.off + sizeof(float) * gi.prow * gi.pcol * g),
-                               (long long)gi.pcol, c0.key.M, c0.key.N});
-          clus2.push_back(row);
-        }
-      }
+    std::map<std::tuple<int, int, int64_t, int64_t, int>, std::vector<int>> geo;
+    for (size_t c = 0; c < cls.size(); ++c)
+      geo[{cls[c].key.M, cls[c].key.N, cls[c].key.r0, cls[c].key.c0, cls[c].col_first ? 1 : 0}].push_back((int)c);
+    for (auto& kv : geo) {
+      GeoInfo gi;
+      const auto& cl = kv.second;
+      for (size_t s = 0; s < cl.size(); s += lambda_group)
+        gi.lgroups.emplace_back(cl.begin() + s, cl.begin() + std::min(cl.size(), s + (size_t)lambda_group));
+      gi.prow = round_up(std::get<0>(kv.first), BM);  // partial[u][v]
+      gi.pcol = round_up(std::get<1>(kv.first), BM);
+      gi.partial = lay.take(sizeof(float) * gi.prow * gi.pcol * gi.lgroups.size());
+      geos.push_back(gi);
     }
   } else {
     plan->epi2 = EPI_STORE_CPLX;
-    for (size_t c = 0; c < cinfo.size(); ++c) {
-      const int trs = (int)(cinfo[c].rows2 / BM), tcs = (int)((cinfo[c].key.M + BN - 1) / BN);
-      for (int r = 0; r < trs; ++r) {
-        std::vector<Work> row;
-        for (int cc = 0; cc < tcs; ++cc) row.push_back(Work{(int)c, (int)c + 1, r, cc, nullptr, 0, 0, 0});
-        clus2.push_back(row);
+  }
+  // stage-2 problem list in launch order (λ-groups contiguous)
+  std::vector<Problem> p2_launch;
+  std::vector<Patch> patch2_launch;
+  if (fast) {
+    for (auto& gi : geos) {
+      for (size_t g = 0; g < gi.lgroups.size(); ++g) {
+        const int pb = (int)p2_launch.size();
+        for (int c : gi.lgroups[g]) {
+          p2_launch.push_back(p2[cls[c].p2]);
+          patch2_launch.push_back(Patch{(int)p2_launch.size() - 1, patch2[cls[c].p2].a, 0});
+        }
+        const ClassInfo& c0 = cls[gi.lgroups[g][0]];
+        const size_t poff = gi.partial + sizeof(float) * gi.prow * gi.pcol * g;
+        // the partial image offset rides in the pointer field until the workspace exists
+        add_tiles(clus2, pb, (int)p2_launch.size(), c0.rows2, c0.col_first ? c0.key.N : c0.key.M,
+                  (float*)(uintptr_t)poff, (long long)gi.pcol, c0.key.M, c0.key.N);
       }
+    }
+  } else {
+    for (auto& c : cls) {
+      const int pb = (int)p2_launch.size();
+      p2_launch.push_back(p2[c.p2]);
+      patch2_launch.push_back(patch2[c.p2]);
+      add_tiles(clus2, pb, pb + 1, c.rows2, c.col_first ? c.key.N : c.key.M);
     }
   }
   std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
 
-  // descriptor block
-  Buf bw1 = reserve(sizeof(Work) * std::max<size_t>(w1.size(), 1));
-  Buf bw2 = reserve(sizeof(Work) * std::max<size_t>(w2.size(), 1));
-  Buf bp1 = reserve(sizeof(Problem) * std::max<size_t>(p1.size(), 1));
-  Buf bp2 = reserve(sizeof(Problem) * std::max<size_t>(p2.size(), 1));
-  Buf bsg = reserve(sizeof(Segment) * std::max<size_t>(segs.size(), 1));
-  Buf biw = reserve(sizeof(double) * std::max<size_t>(inv_order.size(), 1));
+  const size_t o_w1 = lay.take(sizeof(Work) * std::max<size_t>(w1.size(), 1));
+  const size_t o_w2 = lay.take(sizeof(Work) * std::max<size_t>(w2.size(), 1));
+  const size_t o_p1 = lay.take(sizeof(Problem) * std::max<size_t>(p1.size(), 1));
+  const size_t o_p2 = lay.take(sizeof(Problem) * std::max<size_t>(p2_launch.size(), 1));
+  const size_t o_cx = lay.take(sizeof(double) * std::max<size_t>(cx.size(), 1));
+  const size_t o_iw = lay.take(sizeof(double) * std::max<size_t>(inv_order.size(), 1));
+  std::vector<size_t> fb_off;
+  if (!fast)
+    for (size_t g = 0; g < groups.size(); ++g) fb_off.push_back(lay.take(sizeof(float2) * image_rows * image_cols));
 
-  if (hipMalloc(&plan->ws, ws_total) != hipSuccess) {
+  if (hipMalloc(&plan->ws, lay.total) != hipSuccess) {
     delete plan;
-    return fail(ctx, MFTX_ENOMEM, "plan workspace allocation of " + std::to_string(ws_total) + " bytes failed");
+    return fail(ctx, MFTX_ENOMEM, "plan workspace allocation of " + std::to_string(lay.total) + " bytes failed");
   }
-  plan->ws_bytes = ws_total;
   char* base = (char*)plan->ws;
-  for (size_t i = 0; i < p1.size(); ++i) {
-    p1[i].a = base + (size_t)p1[i].a;
-    p1[i].dst = base + (size_t)p1[i].dst;
+  for (auto& pt : patch1) {
+    p1[pt.idx].a = base + pt.a;
+    p1[pt.idx].dst = base + pt.dst;
   }
-  // p2 holds one entry per class, then (fast path) the λ-group copies; each
-  // class owns a distinct seg_begin, which identifies the class of a copy
-  for (size_t i = 0; i < p2.size(); ++i) {
-    size_t c = 0;
-    while (c < cinfo.size() && p2[i].seg_begin != p2[c].seg_begin) ++c;
-    p2[i].a = base + cinfo[c].abuf.off;
-    if (!fast) p2[i].dst = base + cinfo[c].fbuf.off;
+  for (auto& pt : patch2_launch) {
+    p2_launch[pt.idx].a = base + pt.a;
+    p2_launch[pt.idx].dst = pt.dst ? (void*)(base + pt.dst) : nullptr;
   }
   for (auto& w : w2)
-    if (w.partial) w.partial = (float*)(base + (size_t)w.partial);
-
+    if (w.partial) w.partial = (float*)(base + (size_t)(uintptr_t)w.partial);
   std::vector<double> iw(inv_order.size());
   for (size_t i = 0; i < inv_order.size(); ++i) iw[i] = 1.0 / wavelengths[inv_order[i]];
 
-  plan->works1 = (Work*)(base + bw1.off);
-  plan->works2 = (Work*)(base + bw2.off);
-  plan->probs1 = (Problem*)(base + bp1.off);
-  plan->probs2 = (Problem*)(base + bp2.off);
-  plan->segs = (Segment*)(base + bsg.off);
-  plan->inv_wl = (double*)(base + biw.off);
+  plan->works1 = (Work*)(base + o_w1);
+  plan->works2 = (Work*)(base + o_w2);
+  plan->probs1 = (Problem*)(base + o_p1);
+  plan->probs2 = (Problem*)(base + o_p2);
+  plan->chunk_x = (double*)(base + o_cx);
+  plan->inv_wl = (double*)(base + o_iw);
   plan->nwork1 = (int)w1.size();
   plan->nwork2 = (int)w2.size();
   hipError_t e = hipSuccess;
-  if (!w1.empty()) e = hipMemcpy(plan->works1, w1.data(), sizeof(Work) * w1.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !w2.empty()) e = hipMemcpy(plan->works2, w2.data(), sizeof(Work) * w2.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !p1.empty()) e = hipMemcpy(plan->probs1, p1.data(), sizeof(Problem) * p1.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !p2.empty()) e = hipMemcpy(plan->probs2, p2.data(), sizeof(Problem) * p2.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !segs.empty()) e = hipMemcpy(plan->segs, segs.data(), sizeof(Segment) * segs.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !iw.empty()) e = hipMemcpy(plan->inv_wl, iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice);
+  auto up = [&](void* d, const void* s, size_t n) {
+    if (e == hipSuccess && n) e = hipMemcpy(d, s, n, hipMemcpyHostToDevice);
+  };
+  up(plan->works1, w1.data(), sizeof(Work) * w1.size());
+  up(plan->works2, w2.data(), sizeof(Work) * w2.size());
+  up(plan->probs1, p1.data(), sizeof(Problem) * p1.size());
+  up(plan->probs2, p2_launch.data(), sizeof(Problem) * p2_launch.size());
+  up(plan->chunk_x, cx.data(), sizeof(double) * cx.size());
+  up(plan->inv_wl, iw.data(), sizeof(double) * iw.size());
   if (e != hipSuccess) {
     (void)hipFree(plan->ws);
     delete plan;
     return hipfail(ctx, e, "plan descriptor upload");
   }
 
-  for (int t = 0; t < ntiles; ++t) {
-    if (tile_waves[t].empty()) continue;
+  for (auto& kv : tops) {
+    const TileOp& op = kv.second;
     mftx_plan::Phasor ph;
-    ph.tile = tiles[t];
-    ph.nwave = (int)tile_waves[t].size();
-    ph.wave_off = tile_wave_off[t];
-    ph.dst = base + tile_buf[t].off;
-    ph.wave_stride = (long long)packed_bytes(prec, tile_rows[t], (int)tile_kpad[t]);
-    ph.rows_pad = tile_rows[t];
-    ph.kpad = (int)tile_kpad[t];
+    ph.tile = tiles[kv.first.first];
+    ph.trans = kv.first.second;
+    ph.nwave = (int)op.waves.size();
+    ph.wave_off = op.wave_off;
+    ph.dst = base + op.off;
+    ph.wave_stride = (long long)packed_bytes(prec, op.rows, op.kpad);
+    ph.rows_pad = op.rows;
+    ph.kpad = op.kpad;
     plan->phasors.push_back(ph);
   }
   if (fast) {
-    for (auto& gi : ginfo) {
-      const ClassInfo& c0 = cinfo[gi.lgroups[0][0]];
-      plan->reduces.push_back(mftx_plan::Reduce{(float*)(base + gi.partial.off), (long long)(gi.prow * gi.pcol),
-                                                (int)gi.lgroups.size(), c0.key.M, c0.key.N, (long long)gi.pcol,
+    for (auto& gi : geos) {
+      const ClassInfo& c0 = cls[gi.lgroups[0][0]];
+      plan->reduces.push_back(mftx_plan::Reduce{(float*)(base + gi.partial), (long long)(gi.prow * gi.pcol),
+                                                (long long)gi.pcol, (int)gi.lgroups.size(), c0.key.M, c0.key.N,
                                                 c0.key.r0, c0.key.c0});
     }
   } else {
-    for (auto& g : groups) {
+    size_t g = 0;
+    for (auto& kv : groups) {
       mftx_plan::Fallback fb;
-      fb.acc = (float2*)(base + fb_acc[group_fb[g.first]].off);
-      fb.weight = problems[(*g.second[0])[0]].weight;
-      for (auto* mem : g.second) {
-        size_t c = 0;
-        for (; c < cinfo.size(); ++c)
-          if (cinfo[c].members == mem) break;
-        const ClassKey& K = cinfo[c].key;
-        fb.adds.emplace_back((const float2*)(base + cinfo[c].fbuf.off), K.M, K.N, K.r0, K.c0, (long long)K.N);
+      fb.acc = (float2*)(base + fb_off[g++]);
+      fb.weight = problems[cls[kv.second[0]].members[0]].weight;
+      for (int c : kv.second) {
+        const ClassKey& K = cls[c].key;
+        fb.adds.push_back(mftx_plan::Add{(const float2*)(base + cls[c].fbuf), K.M, K.N, K.r0, K.c0});
       }
       plan->fallbacks.push_back(fb);
     }
   }
 
-  // ---- info ------------------------------------------------------------------
+  // ---- info: algorithmic FLOPs are 8 per complex MAC of the cheaper order -------
   double fl = 0, g1 = 0, g2 = 0;
-  for (int i = 0; i < nprob; ++i) {
-    const mftx_problem& P = problems[i];
-    const double m = (double)tiles[P.tile].m, n = (double)tiles[P.tile].n, M = P.M, N = P.N;
-    fl += 8.0 * std::min(M * m * n + M * n * N, m * n * N + M * m * N);
-    g1 += 8.0 * m * n * N;
-    g2 += 8.0 * M * m * N;
-  }
+  for (auto& c : cls)
+    for (int i : c.members) {
+      const mftx_problem& P = problems[i];
+      const double m = (double)tiles[P.tile].m, n = (double)tiles[P.tile].n, M = P.M, N = P.N;
+      fl += 8.0 * std::min(cost_row_first(m, n, M, N), cost_col_first(m, n, M, N));
+      g1 += 8.0 * (c.col_first ? M * m * n : m * n * N);
+      g2 += 8.0 * (c.col_first ? M * n * N : M * m * N);
+    }
   plan->info.flops = fl;
   plan->info.gemm_flops[0] = g1;
   plan->info.gemm_flops[1] = g2;
   double bytes = 0;
-  for (auto& ph : plan->phasors) bytes += (double)ph.wave_stride * ph.nwave * 2;  // write + read
-  for (auto& ci : cinfo) bytes += 2.0 * packed_bytes(prec, ci.rows2, ci.ktot);
+  for (auto& ph : plan->phasors) bytes += 2.0 * (double)ph.wave_stride * ph.nwave;  // write + read
+  for (auto& c : cls) bytes += 2.0 * packed_bytes(prec, c.rows2, c.ktot);
   bytes += 4.0 * image_rows * image_cols;
   plan->info.bytes_hbm_min = bytes;
+  plan->info.coherent_fast = fast ? 1 : 0;
   plan->info.n_stage1 = (int)p1.size();
-  plan->info.n_stage2 = (int)cinfo.size();
+  plan->info.n_stage2 = (int)cls.size();
   plan->info.n_work1 = plan->nwork1;
   plan->info.n_work2 = plan->nwork2;
   plan->info.n_launch = (int)plan->phasors.size() + 2 + (int)plan->reduces.size();
@@ -744,56 +755,74 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   return MFTX_OK;
 }
 
+}  // extern "C"
+
+static hipError_t launch_phasors(const mftx_plan* plan, hipStream_t s) {
+  for (auto& ph : plan->phasors) {
+    hipError_t e = launch_phasor(plan->prec, ph.tile.amp, ph.tile.amp_dtype, ph.tile.ld_amp, ph.tile.opd,
+                                 ph.tile.opd_dtype, ph.tile.ld_opd, (int)ph.tile.m, (int)ph.tile.n, ph.trans,
+                                 plan->inv_wl + ph.wave_off, ph.nwave, ph.dst, ph.wave_stride, ph.rows_pad, ph.kpad, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+static hipError_t launch_reduces(const mftx_plan* plan, void* image, int dtype, int64_t ldi, int accumulate,
+                                 hipStream_t s) {
+  const size_t esz = dtype == 3 ? 8 : 4;
+  bool first = true;
+  for (auto& r : plan->reduces) {
+    int acc = accumulate;
+    if (!accumulate && first && !(r.r0 == 0 && r.c0 == 0 && r.rows == plan->img_rows && r.cols == plan->img_cols)) {
+      hipError_t e = launch_zero(image, esz * plan->img_rows * ldi, s);
+      if (e != hipSuccess) return e;
+      acc = 1;
+    } else if (!accumulate && !first) {
+      acc = 1;
+    }
+    first = false;
+    hipError_t e = launch_reduce_partials(r.partial, r.gstride, r.ngroups, r.rows, r.cols, r.ldp, image, dtype, ldi,
+                                          (int)plan->img_rows, (int)plan->img_cols, (int)r.r0, (int)r.c0, acc, s);
+    if (e != hipSuccess) return e;
+  }
+  if (!accumulate && plan->reduces.empty()) return launch_zero(image, esz * plan->img_rows * ldi, s);
+  return hipSuccess;
+}
+
+static hipError_t launch_fallbacks(const mftx_plan* plan, void* image, int dtype, int64_t ldi, int accumulate,
+                                   hipStream_t s) {
+  const size_t esz = dtype == 3 ? 8 : 4;
+  hipError_t e = hipSuccess;
+  if (!accumulate) e = launch_zero(image, esz * plan->img_rows * ldi, s);
+  for (auto& fb : plan->fallbacks) {
+    if (e == hipSuccess) e = launch_zero(fb.acc, sizeof(float2) * plan->img_rows * plan->img_cols, s);
+    for (auto& a : fb.adds) {
+      // clip F's placement to the image (field.py:282-296)
+      const int64_t or0 = std::max<int64_t>(a.r0, 0), oc0 = std::max<int64_t>(a.c0, 0);
+      const int64_t or1 = std::min<int64_t>(a.r0 + a.M, plan->img_rows), oc1 = std::min<int64_t>(a.c0 + a.N, plan->img_cols);
+      if (or1 <= or0 || oc1 <= oc0 || e != hipSuccess) continue;
+      e = launch_insert(a.F, a.N, (int)(or0 - a.r0), (int)(oc0 - a.c0), (int)(or1 - or0), (int)(oc1 - oc0), fb.acc, 0,
+                        plan->img_cols, (int)or0, (int)oc0, 0, 1.0, s);
+    }
+    if (e == hipSuccess)
+      e = launch_insert(fb.acc, plan->img_cols, 0, 0, (int)plan->img_rows, (int)plan->img_cols, image, dtype, ldi, 0, 0,
+                        1, fb.weight, s);
+  }
+  return e;
+}
+
 static int plan_launch(mftx_plan* plan, void* image, int dtype, int64_t ldi, int accumulate) {
   mftx_ctx* ctx = plan->ctx;
   hipStream_t s = ctx->stream;
-  for (auto& ph : plan->phasors)
-    HIPCHK(ctx, launch_phasor(plan->prec, ph.tile.amp, ph.tile.amp_dtype, ph.tile.ld_amp, ph.tile.opd,
-                              ph.tile.opd_dtype, ph.tile.ld_opd, (int)ph.tile.m, (int)ph.tile.n,
-                              plan->inv_wl + ph.wave_off, ph.nwave, ph.dst, ph.wave_stride, ph.rows_pad, ph.kpad, s));
-  HIPCHK(ctx, launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->works1, plan->nwork1, plan->probs1, plan->segs, s));
-  HIPCHK(ctx, launch_gemm(plan->prec, plan->epi2, plan->works2, plan->nwork2, plan->probs2, plan->segs, s));
-  if (plan->epi2 == EPI_INTENSITY) {
-    bool first = true;
-    for (auto& r : plan->reduces) {
-      // overwrite only on the first reduce that covers the whole image; later
-      // reduces (other output geometries) always accumulate
-      int acc = accumulate;
-      if (!accumulate && !first) acc = 1;
-      if (!accumulate && first) {
-        if (!(r.r0 == 0 && r.c0 == 0 && r.rows == plan->img_rows && r.cols == plan->img_cols)) {
-          HIPCHK(ctx, launch_zero(image, (size_t)(dtype == 3 ? 8 : 4) * plan->img_rows * ldi, s));
-          acc = 1;
-        }
-      }
-      first = false;
-      HIPCHK(ctx, launch_reduce_partials(r.partial, r.gstride, r.ngroups, r.rows, r.cols, r.ldp, image, dtype, ldi,
-                                         (int)plan->img_rows, (int)plan->img_cols, (int)r.r0, (int)r.c0, acc, s));
-    }
-    if (!accumulate && plan->reduces.empty())
-      HIPCHK(ctx, launch_zero(image, (size_t)(dtype == 3 ? 8 : 4) * plan->img_rows * ldi, s));
-  } else {
-    if (!accumulate) HIPCHK(ctx, launch_zero(image, (size_t)(dtype == 3 ? 8 : 4) * plan->img_rows * ldi, s));
-    for (auto& fb : plan->fallbacks) {
-      HIPCHK(ctx, launch_zero(fb.acc, sizeof(float2) * plan->img_rows * plan->img_cols, s));
-      for (auto& a : fb.adds) {
-        const float2* F = std::get<0>(a);
-        const int M = std::get<1>(a), N = std::get<2>(a);
-        const int64_t r0 = std::get<3>(a), c0 = std::get<4>(a);
-        const long long ldF = std::get<5>(a);
-        // clip F's placement to the image
-        const int64_t or0 = std::max<int64_t>(r0, 0), oc0 = std::max<int64_t>(c0, 0);
-        const int64_t or1 = std::min<int64_t>(r0 + M, plan->img_rows), oc1 = std::min<int64_t>(c0 + N, plan->img_cols);
-        if (or1 <= or0 || oc1 <= oc0) continue;
-        HIPCHK(ctx, launch_insert(F, ldF, (int)(or0 - r0), (int)(oc0 - c0), (int)(or1 - or0), (int)(oc1 - oc0),
-                                  fb.acc, 0, plan->img_cols, (int)or0, (int)oc0, 0, 1.0, s));
-      }
-      HIPCHK(ctx, launch_insert(fb.acc, plan->img_cols, 0, 0, (int)plan->img_rows, (int)plan->img_cols, image, dtype,
-                                ldi, 0, 0, 1, fb.weight, s));
-    }
-  }
+  HIPCHK(ctx, launch_phasors(plan, s));
+  HIPCHK(ctx, launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s));
+  HIPCHK(ctx, launch_gemm(plan->prec, plan->epi2, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s));
+  if (plan->epi2 == EPI_INTENSITY) HIPCHK(ctx, launch_reduces(plan, image, dtype, ldi, accumulate, s));
+  else HIPCHK(ctx, launch_fallbacks(plan, image, dtype, ldi, accumulate, s));
   return MFTX_OK;
 }
+
+extern "C" {
 
 int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi, int32_t accumulate) {
   if (!plan || !plan->ctx) return MFTX_EHANDLE;
@@ -805,26 +834,25 @@ int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
   if (!plan->use_graph) return plan_launch(plan, image, image_dtype, ldi, accumulate);
   if (!plan->gexec || plan->g_image != image || plan->g_dtype != image_dtype || plan->g_ld != ldi ||
       plan->g_accum != accumulate) {
-    if (plan->gexec) {
-      (void)hipGraphExecDestroy(plan->gexec);
-      plan->gexec = nullptr;
-    }
-    if (plan->graph) {
-      (void)hipGraphDestroy(plan->graph);
-      plan->graph = nullptr;
-    }
+    if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    plan->gexec = nullptr;
+    plan->graph = nullptr;
     hipStream_t cap = nullptr;
     HIPCHK(ctx, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipStream_t saved = ctx->stream;
     ctx->stream = cap;
-    HIPCHK(ctx, hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    int rc = plan_launch(plan, image, image_dtype, ldi, accumulate);
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    int rc = e == hipSuccess ? plan_launch(plan, image, image_dtype, ldi, accumulate) : MFTX_OK;
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(cap, &g);
+    if (e == hipSuccess) e = hipStreamEndCapture(cap, &g);
     ctx->stream = saved;
     (void)hipStreamDestroy(cap);
-    if (rc) return rc;
-    if (e != hipSuccess) return hipfail(ctx, e, "hipStreamEndCapture");
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hipfail(ctx, e, "graph capture");
     plan->graph = g;
     HIPCHK(ctx, hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0));
     plan->g_image = image;
@@ -836,58 +864,40 @@ int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
   return MFTX_OK;
 }
 
-int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi, int32_t repeats,
-                      double* ms) {
+int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi, int32_t repeats, double* ms) {
   if (!plan || !plan->ctx) return MFTX_EHANDLE;
   mftx_ctx* ctx = plan->ctx;
   if (!image || !ms || repeats <= 0) return fail(ctx, MFTX_EINVAL, "bad profile request");
   if (image_dtype != 1 && image_dtype != 3) return fail(ctx, MFTX_EINVAL, "image must be float32 or float64");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  hipEvent_t ev[5];
-  for (auto& e : ev) HIPCHK(ctx, hipEventCreate(&e));
+  hipEvent_t ev[5] = {};
+  for (auto& x : ev) HIPCHK(ctx, hipEventCreate(&x));
   double acc[4] = {0, 0, 0, 0};
-  int rc = MFTX_OK;
-  for (int r = 0; r < repeats && rc == MFTX_OK; ++r) {
-    hipError_t e = hipEventRecord(ev[0], s);
-    for (auto& ph : plan->phasors)
-      if (e == hipSuccess)
-        e = launch_phasor(plan->prec, ph.tile.amp, ph.tile.amp_dtype, ph.tile.ld_amp, ph.tile.opd, ph.tile.opd_dtype,
-                          ph.tile.ld_opd, (int)ph.tile.m, (int)ph.tile.n, plan->inv_wl + ph.wave_off, ph.nwave,
-                          ph.dst, ph.wave_stride, ph.rows_pad, ph.kpad, s);
+  hipError_t e = hipSuccess;
+  for (int r = 0; r < repeats && e == hipSuccess; ++r) {
+    e = hipEventRecord(ev[0], s);
+    if (e == hipSuccess) e = launch_phasors(plan, s);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s);
-    if (e == hipSuccess) e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->works1, plan->nwork1, plan->probs1, plan->segs, s);
+    if (e == hipSuccess)
+      e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s);
-    if (e == hipSuccess) e = launch_gemm(plan->prec, plan->epi2, plan->works2, plan->nwork2, plan->probs2, plan->segs, s);
+    if (e == hipSuccess)
+      e = launch_gemm(plan->prec, plan->epi2, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s);
     if (e == hipSuccess) e = hipEventRecord(ev[3], s);
-    if (e != hipSuccess) {
-      rc = hipfail(ctx, e, "profile launch");
-      break;
-    }
-    // the detector reduction (or the coherent fallback) via the regular path
-    // would re-run every stage; time only its own launches
-    if (plan->epi2 == EPI_INTENSITY) {
-      for (auto& red : plan->reduces) {
-        e = launch_reduce_partials(red.partial, red.gstride, red.ngroups, red.rows, red.cols, red.ldp, image,
-                                   image_dtype, ldi, (int)plan->img_rows, (int)plan->img_cols, (int)red.r0,
-                                   (int)red.c0, 0, s);
-        if (e != hipSuccess) break;
-      }
-    }
+    if (e == hipSuccess)
+      e = plan->epi2 == EPI_INTENSITY ? launch_reduces(plan, image, image_dtype, ldi, 0, s)
+                                      : launch_fallbacks(plan, image, image_dtype, ldi, 0, s);
     if (e == hipSuccess) e = hipEventRecord(ev[4], s);
     if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
-    if (e != hipSuccess) {
-      rc = hipfail(ctx, e, "profile run");
-      break;
-    }
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
       float t = 0;
-      hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+      e = hipEventElapsedTime(&t, ev[k], ev[k + 1]);
       acc[k] += t;
     }
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  if (rc) return rc;
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  if (e != hipSuccess) return hipfail(ctx, e, "plan profile");
   for (int k = 0; k < 4; ++k) ms[k] = acc[k] / repeats;
   return MFTX_OK;
 }

@@ -21,7 +21,10 @@ enum Epilogue {
 };
 
 // One complex GEMM problem.  Twiddle B[k][j] = cis(alpha * x_k * y_j) with
-// x_k = seg.x0 + (k - seg.kbeg) and y_j = y0 + j; cis(t) = exp(2*pi*i*t).
+// y_j = y0 + j and x_k = chunk_x[chunk_begin + k/BK] + (k % BK): x advances
+// by one per K index inside each BK-wide chunk, and each chunk carries its
+// own starting x (so several K segments with their own coordinate origins
+// can be concatenated); cis(t) = exp(2*pi*i*t).
 struct Problem {
   const void* a;        // packed A operand (row-major, K-blocked)
   long long lda;        // A row stride in scalar elements (float2 for fp32, bf16 for bf16x3)
@@ -29,7 +32,7 @@ struct Problem {
   double y0;            // y of column 0
   int rows, cols;       // valid C extent
   int kpad;             // K (multiple of BK); A is zero beyond the valid K
-  int seg_begin, seg_count;
+  int chunk_begin;      // index of this problem's first entry in chunk_x
   float scale;          // store: C *= scale; intensity: weight applied to |C|^2
   int limit;            // EPI_STORE_PACKED_T: C rows written (K extent of the destination)
   void* dst;            // store destination (EPI_STORE_*)
@@ -38,6 +41,8 @@ struct Problem {
   int pad_;
 };
 
+// A contiguous K range with its own coordinate origin; the host expands
+// segments into the per-chunk chunk_x table.
 struct Segment {
   int kbeg, kend;       // [kbeg, kend), multiples of BK
   double x0;            // x at k = kbeg
@@ -66,11 +71,11 @@ __host__ __device__ inline size_t packed_bytes(int prec, long long rows, int kpa
 
 // launchers (mftx_kernels.hip)
 hipError_t launch_gemm(int prec, int epi, const Work* work, int nwork, const Problem* probs,
-                       const Segment* segs, hipStream_t s);
+                       const double* chunk_x, hipStream_t s);
 hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n,
-                       long long lds, void* dst, long long rows_pad, int kpad, hipStream_t s);
+                       long long lds, int trans, void* dst, long long rows_pad, int kpad, hipStream_t s);
 hipError_t launch_phasor(int prec, const void* amp, int amp_dtype, long long ld_amp,
-                         const void* opd, int opd_dtype, long long ld_opd, int m, int n,
+                         const void* opd, int opd_dtype, long long ld_opd, int m, int n, int trans,
                          const double* inv_wavelengths_dev, int nwave, void* dst,
                          long long dst_wave_stride, long long rows_pad, int kpad, hipStream_t s);
 hipError_t launch_phasor_field(const void* amp, int amp_dtype, long long ld_amp, const void* opd,
