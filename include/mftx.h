@@ -107,6 +107,27 @@ int mftx_insert(mftx_ctx* ctx, const void* field, int64_t ldf, int64_t fr0, int6
 int mftx_multiply(mftx_ctx* ctx, const void* a, int64_t lda, const void* b, int64_t ldb,
                   void* out, int64_t ldo, int64_t h, int64_t w);
 
+/* ---------------------------------------------------------------------------
+ * Phase-retrieval backward pass.  The reference has no gradients; these
+ * two element-wise steps plus mftx_dft2 with conj == 1 (the adjoint MFT)
+ * give dL/d opd and dL/d amp of weighted broadband images, recomputing the
+ * forward fields instead of storing them.
+ * ------------------------------------------------------------------------- */
+
+/* out = scale * w * F elementwise (F, out complex64; w float32/float64, h x wd):
+ * the cotangent weight * dL/dI * F of I = weight * |F|^2. */
+int mftx_weight_field(mftx_ctx* ctx, const void* F, int64_t ldF, const void* w, int w_dtype, int64_t ldw,
+                      double scale, void* out, int64_t ldo, int64_t h, int64_t wd);
+
+/* Accumulate the OPD (and optionally amplitude) gradient of
+ * f = amp * exp(2*pi*i*opd/wavelength) given D = dL/d conj(f) (complex64):
+ *   grad_opd += -(4*pi/wavelength) * amp * Im(conj(D) exp(i theta))
+ *   grad_amp += 2 * Re(conj(D) exp(i theta))          (grad_amp may be NULL)
+ * grad arrays are float64, m x n. */
+int mftx_phase_grad(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd, int opd_dtype,
+                    int64_t ld_opd, int64_t m, int64_t n, double wavelength, const void* D, int64_t ldD,
+                    double* grad_opd, int64_t ld_gopd, double* grad_amp, int64_t ld_gamp);
+
 /* Fill `bytes` bytes at device pointer p with zeros (a kernel, not a memset node). */
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes);
 

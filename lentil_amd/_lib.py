@@ -62,6 +62,9 @@ SIGNATURES = {
                             c_i64, c_i64, c_int, c_dbl]),
     'mftx_multiply': (c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64]),
     'mftx_zero': (c_int, [c_vp, c_vp, c_i64]),
+    'mftx_weight_field': (c_int, [c_vp, c_vp, c_i64, c_vp, c_int, c_i64, c_dbl, c_vp, c_i64, c_i64, c_i64]),
+    'mftx_phase_grad': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, c_i64, c_dbl, c_vp, c_i64,
+                                c_vp, c_i64, c_vp, c_i64]),
     'mftx_plan_psf': (c_int, [c_vp, ctypes.POINTER(MftxTile), ctypes.c_int32, ctypes.POINTER(c_dbl),
                               ctypes.c_int32, ctypes.POINTER(MftxProblem), ctypes.c_int32, c_i64, c_i64,
                               ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(c_vp)]),

@@ -1,0 +1,120 @@
+"""Differentiable broadband PSFs for phase retrieval (no reference counterpart).
+
+``BroadbandPSFModel(amp, diversity, wavelengths, weights, dx, du, z, npix,
+oversample)`` maps an OPD map (and optionally the amplitude) to one image per
+diversity OPD,
+
+    I_d = sum_l w_l |dft2(amp exp(2 pi i (opd + div_d)/l), alpha_l, npix*os, unitary)|^2
+
+which is the reference's broadband loop (docs/user/diffraction.rst:148-160,
+propagate.py:147-242 with a monolithic pupil).  Gradients flow through a
+``torch.autograd.Function`` whose backward pass recomputes the forward
+fields (nothing is stored but the inputs) and applies the adjoint MFT:
+
+    D = dL/d conj(f) = dft2_adjoint(w g F),   g = dL/dI
+    dL/d opd = sum (2 pi / l) (-2) Im(conj(D) f),  dL/d amp = sum 2 Re(conj(D) e^{i theta})
+
+The forward runs as one batched PSFPlan per diversity; the backward uses
+mftx_dft2 (forward and conjugate twiddles), mftx_weight_field and
+mftx_phase_grad.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .batched import PSFPlan
+from .plane import Pupil, device_phasor
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class BroadbandPSFModel:
+    """Multi-diversity broadband PSF model on the device with gradients."""
+
+    def __init__(self, amp, diversity, wavelengths, weights, dx, du, z, npix, oversample=2,
+                 precision='bf16x3', lambda_group=4):
+        self.dev = _lib.device()
+        amp = np.asarray(amp, dtype=float)
+        self.shape = amp.shape
+        self.wavelengths = np.atleast_1d(np.asarray(wavelengths, dtype=float))
+        self.weights = np.broadcast_to(np.asarray(weights, dtype=float), self.wavelengths.shape).copy()
+        self.dx, self.du, self.z = float(dx), float(du), float(z)
+        self.npix, self.oversample = int(npix), int(oversample)
+        self.M = self.npix * self.oversample
+        self.precision = precision
+        self.amp = torch.from_numpy(amp.copy()).to(self.dev)
+        self.div = [torch.as_tensor(np.asarray(d, dtype=float)).to(self.dev) for d in diversity]
+        # a full-grid mask keeps the tile the whole array at offset (0, 0)
+        pupil = Pupil(amplitude=amp, opd=np.zeros(self.shape), mask=np.ones(self.shape), pixelscale=dx,
+                      focal_length=z)
+        self.plans = [PSFPlan(pupil, self.wavelengths, self.weights, pixelscale=du, shape=npix,
+                              oversample=oversample, precision=precision, lambda_group=lambda_group)
+                      for _ in self.div]
+        self.alphas = [(self.dx * self.du) / (wl * self.z * self.oversample) for wl in self.wavelengths]
+
+    def __call__(self, opd, amp=None):
+        """Images (ndiv, M, M) float32 for an OPD tensor (float64, on the device)."""
+        if amp is None:
+            amp = self.amp
+        return _PSFStack.apply(opd, amp, self)
+
+    # ---------------------------------------------------------------- forward
+    def forward_images(self, opd, amp):
+        out = torch.empty((len(self.div), self.M, self.M), dtype=torch.float32, device=self.dev)
+        for d, (div, plan) in enumerate(zip(self.div, self.plans)):
+            plan.set_opd(0, opd + div)
+            plan.set_amplitude(0, amp)
+            plan(out[d])
+        return out
+
+    # --------------------------------------------------------------- backward
+    def backward_grads(self, opd, amp, grad_images, need_amp):
+        lib = _lib.load()
+        ctx = _lib.context()
+        m, n = self.shape
+        M = self.M
+        prec = _lib.precision_code(self.precision)
+        amp64 = amp.detach().to(torch.float64).contiguous()
+        g_opd = torch.zeros((m, n), dtype=torch.float64, device=self.dev)
+        g_amp = torch.zeros((m, n), dtype=torch.float64, device=self.dev) if need_amp else None
+        F = torch.empty((M, M), dtype=torch.complex64, device=self.dev)
+        Y = torch.empty_like(F)
+        D = torch.empty((m, n), dtype=torch.complex64, device=self.dev)
+        gI = grad_images.detach().contiguous()
+        for d, div in enumerate(self.div):
+            tot = (opd.detach() + div).contiguous()
+            for wl, wt, a in zip(self.wavelengths, self.weights, self.alphas):
+                scale = float(np.sqrt(a * a))
+                f = device_phasor(amp64, tot, wl)                       # recompute the field
+                _lib.check(lib.mftx_dft2(ctx, _ptr(f), _lib.DT_C64, m, n, n, _ptr(F), M, M, M, a, a, 0.0, 0.0,
+                                         0.0, 0.0, scale, 0, prec), ctx, 'mftx_dft2')
+                gd = gI[d]
+                _lib.check(lib.mftx_weight_field(ctx, _ptr(F), M, _ptr(gd), _lib.dtype_code(gd), gd.stride(0),
+                                                 float(wt), _ptr(Y), M, M, M), ctx, 'mftx_weight_field')
+                # adjoint MFT: conjugate twiddles from the M x M grid back to m x n
+                _lib.check(lib.mftx_dft2(ctx, _ptr(Y), _lib.DT_C64, M, M, M, _ptr(D), m, n, n, a, a, 0.0, 0.0,
+                                         0.0, 0.0, scale, 1, prec), ctx, 'mftx_dft2 (adjoint)')
+                _lib.check(lib.mftx_phase_grad(ctx, _ptr(amp64), _lib.DT_F64, n, _ptr(tot), _lib.DT_F64, n, m, n,
+                                               float(wl), _ptr(D), n, _ptr(g_opd), n,
+                                               _ptr(g_amp) if need_amp else None, n),
+                           ctx, 'mftx_phase_grad')
+        return g_opd, g_amp
+
+
+class _PSFStack(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, opd, amp, model):
+        ctx.model = model
+        ctx.save_for_backward(opd, amp)
+        return model.forward_images(opd.detach(), amp.detach())
+
+    @staticmethod
+    def backward(ctx, grad_images):
+        opd, amp = ctx.saved_tensors
+        g_opd, g_amp = ctx.model.backward_grads(opd, amp, grad_images, ctx.needs_input_grad[1])
+        return (g_opd.to(opd.dtype) if ctx.needs_input_grad[0] else None,
+                g_amp.to(amp.dtype) if g_amp is not None else None, None)

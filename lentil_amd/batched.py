@@ -210,6 +210,10 @@ class PSFPlan:
         """Overwrite a tile's OPD in place (the plan re-reads it on every call)."""
         self._amp_opd[tile][1].copy_(torch.as_tensor(opd, dtype=torch.float64))
 
+    def set_amplitude(self, tile, amp):
+        """Overwrite a tile's (masked) amplitude in place."""
+        self._amp_opd[tile][0].copy_(torch.as_tensor(amp, dtype=torch.float32))
+
     def close(self):
         if getattr(self, '_plan', None):
             _lib.load().mftx_plan_destroy(self._plan)

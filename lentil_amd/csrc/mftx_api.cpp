@@ -369,6 +369,33 @@ int mftx_multiply(mftx_ctx* ctx, const void* a, int64_t lda, const void* b, int6
   return MFTX_OK;
 }
 
+int mftx_weight_field(mftx_ctx* ctx, const void* F, int64_t ldF, const void* w, int w_dtype, int64_t ldw,
+                      double scale, void* out, int64_t ldo, int64_t h, int64_t wd) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!F || !w || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if (w_dtype != 1 && w_dtype != 3) return fail(ctx, MFTX_EINVAL, "weight must be float32 or float64");
+  if (h < 0 || wd < 0 || ldF < wd || ldw < wd || ldo < wd) return fail(ctx, MFTX_EINVAL, "bad weight shape");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, launch_weight_field(F, ldF, w, w_dtype, ldw, scale, out, ldo, (int)h, (int)wd, ctx->stream));
+  return MFTX_OK;
+}
+
+int mftx_phase_grad(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd, int opd_dtype,
+                    int64_t ld_opd, int64_t m, int64_t n, double wavelength, const void* D, int64_t ldD,
+                    double* grad_opd, int64_t ld_gopd, double* grad_amp, int64_t ld_gamp) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!amp || !opd || !D || !grad_opd) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if ((amp_dtype != 1 && amp_dtype != 3) || (opd_dtype != 1 && opd_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "amp/opd must be float32 or float64");
+  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n || ldD < n || ld_gopd < n || (grad_amp && ld_gamp < n))
+    return fail(ctx, MFTX_EINVAL, "bad gradient shape");
+  if (!(wavelength > 0)) return fail(ctx, MFTX_EINVAL, "wavelength must be positive");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, launch_phase_grad(amp, amp_dtype, ld_amp, opd, opd_dtype, ld_opd, (int)m, (int)n, 1.0 / wavelength, D,
+                                ldD, grad_opd, ld_gopd, grad_amp, ld_gamp, ctx->stream));
+  return MFTX_OK;
+}
+
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes) {
   if (!ctx) return MFTX_EHANDLE;
   if (!p || bytes < 0) return fail(ctx, MFTX_EINVAL, "bad zero request");

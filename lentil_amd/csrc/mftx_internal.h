@@ -91,5 +91,11 @@ hipError_t launch_insert(const void* field, long long ldf, int fr0, int fc0, int
 hipError_t launch_multiply(const void* a, long long lda, const void* b, long long ldb, void* out,
                            long long ldo, int h, int w, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_weight_field(const void* F, long long ldF, const void* w, int w_dtype, long long ldw,
+                               double scale, void* out, long long ldo, int h, int wd, hipStream_t s);
+hipError_t launch_phase_grad(const void* amp, int amp_dtype, long long ld_amp, const void* opd,
+                             int opd_dtype, long long ld_opd, int m, int n, double inv_wl, const void* D,
+                             long long ldD, double* g_opd, long long ldg, double* g_amp, long long ldga,
+                             hipStream_t s);
 
 }  // namespace mftx

@@ -227,6 +227,67 @@ hipError_t launch_insert(const void* field, long long ldf, int fr0, int fc0, int
   return hipGetLastError();
 }
 
+// --------------------------------------------------------------------------
+// phase-retrieval backward pass (no counterpart in the reference)
+// --------------------------------------------------------------------------
+
+// out[r][c] = scale * w[r][c] * F[r][c]: the cotangent w * dL/dI * F of a
+// weighted intensity image (d|F|^2 / d conj(F) = F).
+__global__ void weight_field_kernel(const float2* __restrict__ F, long long ldF, const void* __restrict__ w,
+                                    int w_dtype, long long ldw, double scale, float2* __restrict__ out,
+                                    long long ldo, int h, int wd) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)h * wd) return;
+  const long long r = t / wd, c = t % wd;
+  const double g = (w_dtype == 3 ? ((const double*)w)[r * ldw + c] : (double)((const float*)w)[r * ldw + c]) * scale;
+  const float2 v = F[r * ldF + c];
+  out[r * ldo + c] = make_float2((float)(g * v.x), (float)(g * v.y));
+}
+
+hipError_t launch_weight_field(const void* F, long long ldF, const void* w, int w_dtype, long long ldw, double scale,
+                               void* out, long long ldo, int h, int wd, hipStream_t s) {
+  const long long total = (long long)h * wd;
+  if (total <= 0) return hipSuccess;
+  const int bs = 256;
+  hipLaunchKernelGGL(weight_field_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, (const float2*)F,
+                     ldF, w, w_dtype, ldw, scale, (float2*)out, ldo, h, wd);
+  return hipGetLastError();
+}
+
+// Given D = dL/d conj(f) for f = amp * exp(2 pi i opd / lambda):
+//   grad_opd += (2 pi / lambda) * (-2) Im(conj(D) f)
+//   grad_amp += 2 Re(conj(D) exp(2 pi i opd / lambda))      (if grad_amp)
+// f is recomputed from amp and opd (opd already includes any diversity).
+__global__ void phase_grad_kernel(const void* __restrict__ amp, int amp_dtype, long long ld_amp,
+                                  const void* __restrict__ opd, int opd_dtype, long long ld_opd, int m, int n,
+                                  double inv_wl, const float2* __restrict__ D, long long ldD,
+                                  double* __restrict__ g_opd, long long ldg, double* __restrict__ g_amp,
+                                  long long ldga) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)m * n) return;
+  const long long r = t / n, c = t % n;
+  const double a = amp_dtype == 3 ? ((const double*)amp)[r * ld_amp + c] : (double)((const float*)amp)[r * ld_amp + c];
+  const double o = opd_dtype == 3 ? ((const double*)opd)[r * ld_opd + c] : (double)((const float*)opd)[r * ld_opd + c];
+  const float2 e = cis_turns(o * inv_wl);
+  const float2 d = D[r * ldD + c];
+  // conj(D) * e = (d.x - i d.y)(e.x + i e.y)
+  const double re = (double)d.x * e.x + (double)d.y * e.y;
+  const double im = (double)d.x * e.y - (double)d.y * e.x;
+  g_opd[r * ldg + c] += -4.0 * 3.14159265358979323846 * inv_wl * a * im;
+  if (g_amp) g_amp[r * ldga + c] += 2.0 * re;
+}
+
+hipError_t launch_phase_grad(const void* amp, int amp_dtype, long long ld_amp, const void* opd, int opd_dtype,
+                             long long ld_opd, int m, int n, double inv_wl, const void* D, long long ldD,
+                             double* g_opd, long long ldg, double* g_amp, long long ldga, hipStream_t s) {
+  const long long total = (long long)m * n;
+  if (total <= 0) return hipSuccess;
+  const int bs = 256;
+  hipLaunchKernelGGL(phase_grad_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, amp, amp_dtype, ld_amp,
+                     opd, opd_dtype, ld_opd, m, n, inv_wl, (const float2*)D, ldD, g_opd, ldg, g_amp, ldga);
+  return hipGetLastError();
+}
+
 // out[r][c] = a[r][c] * b[r][c] (complex64), the Field product of field.py:142.
 __global__ void multiply_kernel(const float2* __restrict__ a, long long lda, const float2* __restrict__ b,
                                 long long ldb, float2* __restrict__ out, long long ldo, int h, int w) {
