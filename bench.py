@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 PEAK = {'bf16x3': 2.5e15, 'fp32': 157.3e12}   # dense MFMA peaks of MI355X (FLOP/s)
 MFMA_PASSES = {'bf16x3': 3, 'fp32': 1}        # executed MFMA work / algorithmic work
+HBM_PEAK = 8e12                               # MI355X HBM3E bytes/s
 
 
 def parse():
@@ -40,7 +41,28 @@ def parse():
     ap.add_argument('--lambda-group', type=int, default=4)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the FP64 error check')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain per-wavelength FP32 path')
+    ap.add_argument('--copy-mib', type=int, default=0,
+                    help='after the timed steps, run one plain device copy of this many MiB (a known-size '
+                         'kernel that scales hardware byte counters)')
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'broadband_512_stage2_hbm_traffic.md')
+
+
+def measured_traffic(precision, lambda_group):
+    """HBM bytes per stage-2 launch measured with rocprofv3 --pmc (FETCH_SIZE
+    and WRITE_SIZE, scaled by a plain copy of known size), read from the
+    committed counter summary; None when it holds no figure for this
+    configuration."""
+    import re
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(rf'^traffic_bytes_per_launch precision={precision} lambda_group={lambda_group}: (\d+)', text,
+                  re.M)
+    return int(m.group(1)) if m else None
 
 
 def cpu_baseline(cfg, budget_s=25.0):
@@ -64,7 +86,15 @@ def cpu_baseline(cfg, budget_s=25.0):
         if len(times) >= 5 or time.perf_counter() - t_start > budget_s:
             break
     med = float(np.median(times))
-    return len(wl) / med, img, threads, f'full {len(wl)}-wavelength image, median of {len(times)} runs ({med:.2f} s each)'
+    model = ''
+    try:
+        with open('/proc/cpuinfo') as fh:
+            model = next((l.split(':', 1)[1].strip() for l in fh if l.startswith('model name')), '')
+    except OSError:
+        pass
+    sample = (f'complex128 oracle of the reference loop, full {len(wl)}-wavelength image, median of {len(times)} '
+              f'runs ({med:.2f} s each); {threads} BLAS threads of {os.cpu_count()} logical CPUs ({model})')
+    return len(wl) / med, img, threads, sample
 
 
 def main():
@@ -97,6 +127,12 @@ def main():
     psfs_per_s = nwave * args.steps / dt
     image = out.double().cpu().numpy()
     print(f'[bench] {args.precision}: {ms_step:.3f} ms/image, {psfs_per_s:.1f} PSFs/s', file=sys.stderr, flush=True)
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # dominant kernel: stage-2 GEMM, HIP events on the plan's stream
     stage_ms = plan.profile(torch.empty_like(out), repeats=5)
@@ -104,11 +140,17 @@ def main():
     ach = flops2 / (stage_ms['stage2'] * 1e-3)
     total_flops = plan.info['flops']
     roofline = {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK[args.precision] / 1e12, 'unit': 'TFLOP/s',
-                'frac': ach / PEAK[args.precision], 'traffic': None,
+                'frac': ach / PEAK[args.precision], 'traffic': measured_traffic(args.precision, args.lambda_group),
+                'traffic_source': os.path.relpath(TRAFFIC_FILE, ROOT),
                 'kernel': 'gemm_kernel<bf16x3|fp32, intensity> (stage 2)',
                 'algorithmic_flops_per_launch': flops2, 'avg_launch_ms': stage_ms['stage2'],
                 'mfma_pipe_frac': ach * MFMA_PASSES[args.precision] / PEAK[args.precision]}
     print(f'[bench] stages ms {stage_ms}', file=sys.stderr, flush=True)
+    # HBM-bound field kernels: algorithmic bytes / event-timed duration
+    field_kernels = {k: {'ms': stage_ms[k], 'bytes': plan.info[f'bytes_{k}'],
+                         'gb_per_s': plan.info[f'bytes_{k}'] / (stage_ms[k] * 1e-3) / 1e9,
+                         'frac_of_hbm_peak': plan.info[f'bytes_{k}'] / (stage_ms[k] * 1e-3) / HBM_PEAK}
+                     for k in ('phasor', 'reduce')}
 
     extra = {}
     if not args.no_plain:
@@ -167,6 +209,7 @@ def main():
         'tflops_algorithmic': total_flops * args.steps / dt / 1e12,
         'stage_ms': stage_ms,
         'roofline': roofline,
+        'field_kernels': field_kernels,
         'cpu_baseline': cpu,
     }
     line.update(extra)

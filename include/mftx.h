@@ -169,6 +169,8 @@ typedef struct {
   double flops;          /* algorithmic FLOPs of one execute (8 per complex MAC) */
   double gemm_flops[2];  /* per GEMM stage */
   double bytes_hbm_min;  /* minimum HBM bytes of one execute (phasor + intermediates + image) */
+  double bytes_phasor;   /* algorithmic bytes of the phasor launches (amp/opd read once, operands written) */
+  double bytes_reduce;   /* algorithmic bytes of the detector reductions (partials read, image written) */
   int32_t n_launch;      /* kernel launches per execute */
   int32_t coherent_fast; /* 1 if every coherent group runs as one K-concatenated GEMM */
   int32_t n_stage1, n_stage2, n_work1, n_work2;
@@ -176,12 +178,17 @@ typedef struct {
 
 /* Build a plan; validates every problem (MFTX_EINVAL on a bad index or shape)
  * and allocates plan-owned workspace.  Tile pointers are kept and re-read on
- * every execute, so amp/opd contents may change between executes. */
+ * every execute, so amp/opd contents may change between executes.
+ * Schedule knobs (no reference counterpart; lentil_amd.PSFPlan can pick
+ * them by timing candidates): lambda_group = wavelengths whose |E|^2 one
+ * GEMM workgroup sums in registers (0 = 8); tile_n = C columns per GEMM
+ * workgroup, 64 (16-wide K steps) or 128 (32-wide K steps; every tile's K is
+ * padded to a multiple of 32) (0 = 128). */
 int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles,
                   const double* wavelengths, int32_t nwave,
                   const mftx_problem* problems, int32_t nprob,
                   int64_t image_rows, int64_t image_cols, int32_t precision,
-                  int32_t lambda_group, mftx_plan** out);
+                  int32_t lambda_group, int32_t tile_n, mftx_plan** out);
 /* image: device float32 (dtype 1) or float64 (dtype 3), image_rows x image_cols,
  * row stride ldi.  accumulate != 0 adds to image (Wavefront.insert), else overwrites. */
 int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi,

@@ -42,6 +42,7 @@ class MftxProblem(ctypes.Structure):
 
 class MftxPlanInfo(ctypes.Structure):
     _fields_ = [('flops', c_dbl), ('gemm_flops', c_dbl * 2), ('bytes_hbm_min', c_dbl),
+                ('bytes_phasor', c_dbl), ('bytes_reduce', c_dbl),
                 ('n_launch', ctypes.c_int32), ('coherent_fast', ctypes.c_int32),
                 ('n_stage1', ctypes.c_int32), ('n_stage2', ctypes.c_int32),
                 ('n_work1', ctypes.c_int32), ('n_work2', ctypes.c_int32)]
@@ -67,7 +68,7 @@ SIGNATURES = {
                                 c_vp, c_i64, c_vp, c_i64]),
     'mftx_plan_psf': (c_int, [c_vp, ctypes.POINTER(MftxTile), ctypes.c_int32, ctypes.POINTER(c_dbl),
                               ctypes.c_int32, ctypes.POINTER(MftxProblem), ctypes.c_int32, c_i64, c_i64,
-                              ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(c_vp)]),
+                              ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(c_vp)]),
     'mftx_plan_execute': (c_int, [c_vp, c_vp, ctypes.c_int32, c_i64, ctypes.c_int32]),
     'mftx_plan_set_graph': (c_int, [c_vp, ctypes.c_int32]),
     'mftx_plan_get_info': (c_int, [c_vp, ctypes.POINTER(MftxPlanInfo)]),

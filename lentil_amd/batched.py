@@ -125,6 +125,12 @@ def plan_geometry(pupil, wavelengths, weights=None, pixelscale=None, shape=None,
     return dict(shape_out=shape_out, wavelengths=wavelengths, weights=weights, tiles=tiles, problems=problems)
 
 
+# (lambda_group, tile_n) chosen by PSFPlan(autotune=True), keyed by the
+# shapes that decide the schedule; lives for the process.
+_TUNED = {}
+_CANDIDATES = [(lg, tn) for tn in (64, 128) for lg in (4, 8, 16)]
+
+
 class PSFPlan:
     """Weighted broadband (and segmented) PSF accumulated on the device.
 
@@ -132,12 +138,16 @@ class PSFPlan:
     ``Pupil`` (2-D mask: one tile; 3-D mask: one tile per segment), ``tilts``
     are Tilt/DispersiveTilt planes applied after the pupil.  ``precision`` is
     'fp32' or 'bf16x3'; ``lambda_group`` wavelengths share one output tile
-    accumulation in registers; ``use_graph`` replays the launches from a
-    hipGraph after the first call.
+    accumulation in registers; ``tile_n`` is the GEMM workgroup width (64 or
+    128 columns, 0 = library default); ``autotune`` times every
+    (lambda_group, tile_n) candidate on this device and keeps the fastest
+    (remembered per problem shape for the process); ``use_graph`` replays the
+    launches from a hipGraph after the first call.
     """
 
     def __init__(self, pupil, wavelengths, weights=None, pixelscale=None, shape=None, prop_shape=None,
-                 oversample=2, mask=None, tilts=(), precision='bf16x3', lambda_group=4, use_graph=True):
+                 oversample=2, mask=None, tilts=(), precision='bf16x3', lambda_group=8, tile_n=0,
+                 use_graph=True, autotune=False):
         geo = plan_geometry(pupil, wavelengths, weights, pixelscale, shape, prop_shape, oversample, mask, tilts)
         self.wavelengths = geo['wavelengths']
         self.weights = geo['weights']
@@ -145,6 +155,7 @@ class PSFPlan:
         self.oversample = oversample
         self.shape_out = geo['shape_out']
         self.tile_shapes = [a.shape for a, _ in geo['tiles']]
+        self._plan = None
 
         # device tiles (kept alive: the plan holds raw pointers)
         dev = _lib.device()
@@ -161,22 +172,61 @@ class PSFPlan:
                                      p['out_r0'], p['out_c0'], p['weight']) for p in geo['problems']]
         self.n_problems = len(problems)
         self.problems = geo['problems']
-        ctx = _lib.context()
-        lib = _lib.load()
-        tarr = (_lib.MftxTile * len(tiles))(*tiles)
-        warr = (ctypes.c_double * len(self.wavelengths))(*self.wavelengths)
-        parr = (_lib.MftxProblem * max(len(problems), 1))(*problems)
-        h = ctypes.c_void_p()
-        rc = lib.mftx_plan_psf(ctx, tarr, len(tiles), warr, len(self.wavelengths), parr, len(problems),
-                               self.shape_out[0], self.shape_out[1], _lib.precision_code(precision),
-                               int(lambda_group), ctypes.byref(h))
-        _lib.check(rc, ctx, 'mftx_plan_psf')
-        self._plan = h.value
-        lib.mftx_plan_set_graph(self._plan, 1 if use_graph else 0)
+        self._tarr = (_lib.MftxTile * len(tiles))(*tiles)
+        self._warr = (ctypes.c_double * len(self.wavelengths))(*self.wavelengths)
+        self._parr = (_lib.MftxProblem * max(len(problems), 1))(*problems)
+        self._ntiles = len(tiles)
+
+        if autotune:
+            key = (precision, self.shape_out, tuple(self.tile_shapes), len(self.wavelengths),
+                   tuple((p['tile'], p['M'], p['N'], p['group']) for p in self.problems))
+            if key not in _TUNED:
+                _TUNED[key] = self._autotune()
+            lambda_group, tile_n = _TUNED[key]
+        self.lambda_group, self.tile_n = int(lambda_group), int(tile_n)
+        self._plan = self._build(self.lambda_group, self.tile_n)
+        _lib.load().mftx_plan_set_graph(self._plan, 1 if use_graph else 0)
         info = _lib.MftxPlanInfo()
-        lib.mftx_plan_get_info(self._plan, ctypes.byref(info))
+        _lib.load().mftx_plan_get_info(self._plan, ctypes.byref(info))
         self.info = {k: (list(getattr(info, k)) if k == 'gemm_flops' else getattr(info, k))
                      for k, _ in _lib.MftxPlanInfo._fields_}
+
+    def _build(self, lambda_group, tile_n):
+        ctx = _lib.context()
+        h = ctypes.c_void_p()
+        rc = _lib.load().mftx_plan_psf(ctx, self._tarr, self._ntiles, self._warr, len(self.wavelengths), self._parr,
+                                       self.n_problems, self.shape_out[0], self.shape_out[1],
+                                       _lib.precision_code(self.precision), int(lambda_group), int(tile_n),
+                                       ctypes.byref(h))
+        _lib.check(rc, ctx, 'mftx_plan_psf')
+        return h.value
+
+    def _autotune(self, repeats=5):
+        """Time each (lambda_group, tile_n) candidate eagerly on the current
+        stream and return the fastest; self.tuning keeps the timings (ms)."""
+        lib = _lib.load()
+        ctx = _lib.context()
+        img = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
+        times = {}
+        for lg, tn in _CANDIDATES:
+            h = self._build(lg, tn)
+            try:
+                lib.mftx_plan_set_graph(h, 0)
+                for _ in range(2):
+                    _lib.check(lib.mftx_plan_execute(h, ctypes.c_void_p(img.data_ptr()), _lib.DT_F32,
+                                                     img.stride(0), 0), ctx, 'mftx_plan_execute')
+                start = torch.cuda.Event(enable_timing=True)
+                stop = torch.cuda.Event(enable_timing=True)
+                start.record()
+                for _ in range(repeats):
+                    lib.mftx_plan_execute(h, ctypes.c_void_p(img.data_ptr()), _lib.DT_F32, img.stride(0), 0)
+                stop.record()
+                stop.synchronize()
+                times[(lg, tn)] = start.elapsed_time(stop) / repeats
+            finally:
+                lib.mftx_plan_destroy(h)
+        self.tuning = times
+        return min(times, key=times.get)
 
     def __call__(self, out=None, accumulate=False):
         """Compute the image; returns ``out`` (a float32/float64 device tensor)."""

@@ -59,9 +59,9 @@ std::vector<Work> xcd_order(const std::vector<std::vector<Work>>& clusters) {
 }
 
 // Works covering a C of rows_tiled x cols for problems [pb, pe), one cluster per tile row.
-void add_tiles(std::vector<std::vector<Work>>& clusters, int pb, int pe, int64_t rows_tiled, int64_t cols,
+void add_tiles(std::vector<std::vector<Work>>& clusters, int pb, int pe, int64_t rows_tiled, int64_t cols, int bn,
                float* partial = nullptr, long long ldp = 0, int prows = 0, int pcols = 0) {
-  const int tr = (int)(rows_tiled / BM), tc = (int)((cols + BN - 1) / BN);
+  const int tr = (int)(rows_tiled / BM), tc = (int)((cols + bn - 1) / bn);
   for (int r = 0; r < tr; ++r) {
     std::vector<Work> row;
     for (int c = 0; c < tc; ++c) row.push_back(Work{pb, pe, r, c, partial, ldp, prows, pcols});
@@ -92,6 +92,7 @@ struct mftx_ctx {
 struct mftx_plan {
   mftx_ctx* ctx = nullptr;
   int prec = 0;
+  int bn = BN_DEFAULT;  // C columns per GEMM workgroup
   int64_t img_rows = 0, img_cols = 0;
   void* ws = nullptr;  // workspace: operands, intermediates, partial images, descriptors
   Work* works1 = nullptr;
@@ -189,10 +190,11 @@ static int upload_desc(mftx_ctx* ctx, const void* src, size_t bytes, void** dev)
   return MFTX_OK;
 }
 
-// One GEMM launch of a single problem covering rows_tiled x p.cols of C.
-static int run_single(mftx_ctx* ctx, int prec, int epi, Problem p, double x0, int64_t rows_tiled) {
+// One GEMM launch of a single problem covering rows_tiled x p.cols of C with
+// bn-column workgroups (p.kpad a multiple of k_granule(bn)).
+static int run_single(mftx_ctx* ctx, int prec, int epi, int bn, Problem p, double x0, int64_t rows_tiled) {
   std::vector<std::vector<Work>> clusters;
-  add_tiles(clusters, 0, 1, rows_tiled, p.cols);
+  add_tiles(clusters, 0, 1, rows_tiled, p.cols, bn);
   const std::vector<Work> works = xcd_order(clusters);
   std::vector<double> cx;
   push_chunks(cx, p.kpad, x0);
@@ -207,7 +209,7 @@ static int run_single(mftx_ctx* ctx, int prec, int epi, Problem p, double x0, in
   int rc = upload_desc(ctx, host.data(), host.size(), &dev);
   if (rc) return rc;
   char* d = (char*)dev;
-  HIPCHK(ctx, launch_gemm(prec, epi, (const Work*)(d + off_w), (int)works.size(), (const Problem*)d,
+  HIPCHK(ctx, launch_gemm(prec, epi, bn, (const Work*)(d + off_w), (int)works.size(), (const Problem*)d,
                           (const double*)(d + off_cx), ctx->stream));
   return MFTX_OK;
 }
@@ -288,9 +290,13 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   const int64_t c1 = col_first ? M : N;                           // stage-1 output columns
   const int64_t c2 = col_first ? N : M;                           // stage-2 output columns
   const int64_t rows1 = round_up(r1, BM);
-  const int kpad1 = (int)round_up(k1, BK);
   const int64_t rows2 = round_up(c1, BM);
-  const int kpad2 = (int)round_up(r1, BK);
+  // 128-column tiles when the larger stage still gives every CU a workgroup
+  const int64_t wide = std::max((rows1 / BM) * ((c1 + 127) / 128), (rows2 / BM) * ((c2 + 127) / 128));
+  const int bn = wide >= 256 ? 128 : 64;
+  const int kg = k_granule(bn);
+  const int kpad1 = (int)round_up(k1, kg);
+  const int kpad2 = (int)round_up(r1, kg);
   const size_t b1 = packed_bytes(prec, rows1, kpad1);
   const size_t b2 = packed_bytes(prec, rows2, kpad2);
   int rc = ensure_ws(ctx, (size_t)round_up((int64_t)b1, 256) + b2);
@@ -314,7 +320,7 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   p1.limit = kpad2;
   p1.dst = a2;
   p1.ldd = packed_lda(prec, kpad2);
-  rc = run_single(ctx, prec, EPI_STORE_PACKED_T, p1, col_first ? xr0 : xc0, rows1);
+  rc = run_single(ctx, prec, EPI_STORE_PACKED_T, bn, p1, col_first ? xr0 : xc0, rows1);
   if (rc) return rc;
 
   Problem p2 = {};
@@ -329,7 +335,7 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   p2.dst = F;
   p2.ldd = ldF;
   p2.trans = col_first ? 0 : 1;  // C is F (column-first) or F^T (row-first)
-  return run_single(ctx, prec, EPI_STORE_CPLX, p2, col_first ? xc0 : xr0, rows2);
+  return run_single(ctx, prec, EPI_STORE_CPLX, bn, p2, col_first ? xc0 : xr0, rows2);
 }
 
 int mftx_phasor(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd, int opd_dtype,
@@ -451,14 +457,17 @@ extern "C" {
 
 int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const double* wavelengths, int32_t nwave,
                   const mftx_problem* problems, int32_t nprob, int64_t image_rows, int64_t image_cols,
-                  int32_t precision, int32_t lambda_group, mftx_plan** out) {
+                  int32_t precision, int32_t lambda_group, int32_t tile_n, mftx_plan** out) {
   if (!ctx) return MFTX_EHANDLE;
   if (!out || !tiles || !wavelengths || (!problems && nprob > 0)) return fail(ctx, MFTX_EINVAL, "null argument");
   *out = nullptr;
   if (ntiles <= 0 || nwave <= 0 || nprob < 0) return fail(ctx, MFTX_EINVAL, "empty plan");
   if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3) return fail(ctx, MFTX_EINVAL, "unknown precision");
   if (image_rows <= 0 || image_cols <= 0) return fail(ctx, MFTX_EINVAL, "bad image shape");
-  if (lambda_group <= 0) lambda_group = 4;
+  if (lambda_group <= 0) lambda_group = 8;
+  if (tile_n == 0) tile_n = BN_DEFAULT;
+  if (!valid_bn(tile_n)) return fail(ctx, MFTX_EINVAL, "tile_n must be 0, 64 or 128");
+  const int kg = k_granule(tile_n);
   for (int t = 0; t < ntiles; ++t) {
     const mftx_tile& T = tiles[t];
     if (!T.amp || !T.opd || T.m <= 0 || T.n <= 0 || T.m > (1 << 20) || T.n > (1 << 20) || T.ld_amp < T.n ||
@@ -528,7 +537,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     const bool tr = kv.first.second == 1;
     TileOp op;
     op.rows = round_up(tr ? T.n : T.m, BM);
-    op.kpad = (int)round_up(tr ? T.m : T.n, BK);
+    op.kpad = (int)round_up(tr ? T.m : T.n, kg);
     op.wave_off = (int)inv_order.size();
     op.waves = v;
     for (size_t j = 0; j < v.size(); ++j) {
@@ -545,7 +554,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     for (int i : c.members) {
       const mftx_tile& T = tiles[problems[i].tile];
       c.kbeg.push_back(k);
-      k += (int)round_up(c.col_first ? T.n : T.m, BK);
+      k += (int)round_up(c.col_first ? T.n : T.m, kg);
     }
     c.ktot = k;
     c.abuf = lay.take(packed_bytes(prec, c.rows2, c.ktot));
@@ -567,7 +576,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       const mftx_problem& P = problems[c.members[j]];
       const mftx_tile& T = tiles[P.tile];
       const TileOp& op = tops[{P.tile, cf ? 1 : 0}];
-      const int kp2 = (int)round_up(cf ? T.n : T.m, BK);
+      const int kp2 = (int)round_up(cf ? T.n : T.m, kg);
       Problem q = {};
       q.lda = packed_lda(prec, op.kpad);
       q.alpha = -(cf ? P.alpha_r : P.alpha_c);
@@ -583,7 +592,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       const size_t elem = prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(uint16_t);
       patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)op.slot.at(P.wave) * packed_bytes(prec, op.rows, op.kpad),
                              c.abuf + (size_t)c.kbeg[j] * elem});
-      add_tiles(clus1, (int)p1.size(), (int)p1.size() + 1, op.rows, q.cols);
+      add_tiles(clus1, (int)p1.size(), (int)p1.size() + 1, op.rows, q.cols, tile_n);
       p1.push_back(q);
     }
     // stage 2, one problem per class (K concatenated over members)
@@ -598,7 +607,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     for (size_t j = 0; j < c.members.size(); ++j) {
       const mftx_problem& P = problems[c.members[j]];
       const mftx_tile& T = tiles[P.tile];
-      push_chunks(cx, (int)round_up(cf ? T.n : T.m, BK), cf ? -centre(T.n) + P.offset_c : -centre(T.m) + P.offset_r);
+      push_chunks(cx, (int)round_up(cf ? T.n : T.m, kg), cf ? -centre(T.n) + P.offset_c : -centre(T.m) + P.offset_r);
     }
     q.trans = cf ? 0 : 1;
     const double unit = std::sqrt(std::fabs(K.ar * K.ac));  // unitary scale (fourier.py:100-101)
@@ -616,6 +625,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   mftx_plan* plan = new mftx_plan();
   plan->ctx = ctx;
   plan->prec = prec;
+  plan->bn = tile_n;
   plan->img_rows = image_rows;
   plan->img_cols = image_cols;
 
@@ -654,7 +664,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
         const ClassInfo& c0 = cls[gi.lgroups[g][0]];
         const size_t poff = gi.partial + sizeof(float) * gi.prow * gi.pcol * g;
         // the partial image offset rides in the pointer field until the workspace exists
-        add_tiles(clus2, pb, (int)p2_launch.size(), c0.rows2, c0.col_first ? c0.key.N : c0.key.M,
+        add_tiles(clus2, pb, (int)p2_launch.size(), c0.rows2, c0.col_first ? c0.key.N : c0.key.M, tile_n,
                   (float*)(uintptr_t)poff, (long long)gi.pcol, c0.key.M, c0.key.N);
       }
     }
@@ -663,7 +673,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       const int pb = (int)p2_launch.size();
       p2_launch.push_back(p2[c.p2]);
       patch2_launch.push_back(patch2[c.p2]);
-      add_tiles(clus2, pb, pb + 1, c.rows2, c.col_first ? c.key.N : c.key.M);
+      add_tiles(clus2, pb, pb + 1, c.rows2, c.col_first ? c.key.N : c.key.M, tile_n);
     }
   }
   std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
@@ -772,6 +782,17 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   for (auto& c : cls) bytes += 2.0 * packed_bytes(prec, c.rows2, c.ktot);
   bytes += 4.0 * image_rows * image_cols;
   plan->info.bytes_hbm_min = bytes;
+  double bph = 0, bre = 0;
+  for (auto& ph : plan->phasors)
+    bph += (double)ph.tile.m * ph.tile.n * ((ph.tile.amp_dtype == MFTX_DTYPE_F64 ? 8.0 : 4.0) + (ph.tile.opd_dtype == MFTX_DTYPE_F64 ? 8.0 : 4.0)) +
+           (double)ph.wave_stride * ph.nwave;
+  for (auto& r : plan->reduces) {
+    const double vr = (double)std::max<int64_t>(0, std::min<int64_t>(r.r0 + r.rows, image_rows) - std::max<int64_t>(r.r0, 0));
+    const double vc = (double)std::max<int64_t>(0, std::min<int64_t>(r.c0 + r.cols, image_cols) - std::max<int64_t>(r.c0, 0));
+    bre += 4.0 * r.ngroups * vr * vc + 4.0 * vr * vc;
+  }
+  plan->info.bytes_phasor = bph;
+  plan->info.bytes_reduce = bre;
   plan->info.coherent_fast = fast ? 1 : 0;
   plan->info.n_stage1 = (int)p1.size();
   plan->info.n_stage2 = (int)cls.size();
@@ -842,8 +863,8 @@ static int plan_launch(mftx_plan* plan, void* image, int dtype, int64_t ldi, int
   mftx_ctx* ctx = plan->ctx;
   hipStream_t s = ctx->stream;
   HIPCHK(ctx, launch_phasors(plan, s));
-  HIPCHK(ctx, launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s));
-  HIPCHK(ctx, launch_gemm(plan->prec, plan->epi2, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s));
+  HIPCHK(ctx, launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->bn, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s));
+  HIPCHK(ctx, launch_gemm(plan->prec, plan->epi2, plan->bn, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s));
   if (plan->epi2 == EPI_INTENSITY) HIPCHK(ctx, launch_reduces(plan, image, dtype, ldi, accumulate, s));
   else HIPCHK(ctx, launch_fallbacks(plan, image, dtype, ldi, accumulate, s));
   return MFTX_OK;
@@ -907,10 +928,10 @@ int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
     if (e == hipSuccess) e = launch_phasors(plan, s);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s);
     if (e == hipSuccess)
-      e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s);
+      e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->bn, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s);
     if (e == hipSuccess) e = hipEventRecord(ev[2], s);
     if (e == hipSuccess)
-      e = launch_gemm(plan->prec, plan->epi2, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s);
+      e = launch_gemm(plan->prec, plan->epi2, plan->bn, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s);
     if (e == hipSuccess) e = hipEventRecord(ev[3], s);
     if (e == hipSuccess)
       e = plan->epi2 == EPI_INTENSITY ? launch_reduces(plan, image, image_dtype, ldi, 0, s)

@@ -5,11 +5,18 @@
 
 namespace mftx {
 
-// Tile geometry of the complex GEMM C = A . B (A data, B generated twiddles).
-constexpr int BM = 128;   // C rows per workgroup (A rows)
-constexpr int BN = 128;   // C cols per workgroup (twiddle columns)
-constexpr int BK = 16;    // K per pipeline stage
-constexpr int NTHREADS = 256;
+// Tile geometry of the complex GEMM C = A . B (A data, B generated twiddles):
+// 2 x WN waves per workgroup, each wave a 64 x 32 block of C; C tile
+// BM x BN with BN = 32 * WN, WN = 2 (BN 64) or 4 (BN 128).
+constexpr int WAVES_M = 2;
+constexpr int BM = 64 * WAVES_M;   // C rows per workgroup (A rows)
+constexpr int BK = 16;             // K block: granule of the packed layout and of chunk_x
+constexpr int BN_DEFAULT = 128;    // C cols per workgroup unless a plan picks another
+inline bool valid_bn(int bn) { return bn == 64 || bn == 128; }
+// K per GEMM pipeline stage for a tile width (64 columns: 16, 128 columns: 32);
+// every GEMM problem's kpad is a multiple of it.
+inline int k_granule(int bn) { return bn == 128 ? 32 : 16; }
+static_assert(BM == 128, "staging assumes 128 A rows per workgroup");
 
 enum Precision { PREC_FP32 = 0, PREC_BF16X3 = 1 };
 
@@ -31,7 +38,7 @@ struct Problem {
   double alpha;         // signed twiddle scale in turns per unit of x*y
   double y0;            // y of column 0
   int rows, cols;       // valid C extent
-  int kpad;             // K (multiple of BK); A is zero beyond the valid K
+  int kpad;             // K (multiple of the launch's k_granule); A is zero beyond the valid K
   int chunk_begin;      // index of this problem's first entry in chunk_x
   float scale;          // store: C *= scale; intensity: weight applied to |C|^2
   int limit;            // EPI_STORE_PACKED_T: C rows written (K extent of the destination)
@@ -70,7 +77,8 @@ __host__ __device__ inline size_t packed_bytes(int prec, long long rows, int kpa
 }
 
 // launchers (mftx_kernels.hip)
-hipError_t launch_gemm(int prec, int epi, const Work* work, int nwork, const Problem* probs,
+// bn: C columns per workgroup (64 or 128, see valid_bn).
+hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, const Problem* probs,
                        const double* chunk_x, hipStream_t s);
 hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n,
                        long long lds, int trans, void* dst, long long rows_pad, int kpad, hipStream_t s);

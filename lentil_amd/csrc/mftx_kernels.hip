@@ -85,42 +85,55 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
   return dtype == 3 ? ((const double*)p)[idx] : (double)((const float*)p)[idx];
 }
 
-// One thread per (row, 16-wide K block, wavelength); writes the packed
-// operand of the m x n tile, transposed (n x m) when trans.
+// One thread per (row, 16-wide K block, group of PH_WAVES wavelengths):
+// the 16 amp/opd values are read once and the packed operand of the m x n
+// tile (n x m when trans) is written for each wavelength of the group.
+// Consecutive threads take consecutive source columns (K blocks when not
+// trans, rows of the transposed operand when trans), so the source reads of
+// a wave are contiguous.
+constexpr int PH_WAVES = 8;
+
 __global__ void phasor_kernel(int prec, const void* __restrict__ amp, int amp_dtype, long long ld_amp,
                               const void* __restrict__ opd, int opd_dtype, long long ld_opd, int m, int n,
                               int trans, const double* __restrict__ inv_wl, int nwave, void* __restrict__ dst,
                               long long wave_stride, long long rows_pad, int kpad) {
   const int kblocks = kpad / 16;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long per_wave = rows_pad * kblocks;
-  if (t >= per_wave * nwave) return;
-  const int w = (int)(t / per_wave);
-  const long long r = t % per_wave;
-  const long long row = r / kblocks;
-  const int kb = (int)(r % kblocks);
-  const double iw = inv_wl[w];
-  float2 v[16];
+  const long long per_group = rows_pad * kblocks;
+  const int ngroups = (nwave + PH_WAVES - 1) / PH_WAVES;
+  if (t >= per_group * ngroups) return;
+  const int g = (int)(t / per_group);
+  const long long r = t % per_group;
+  const long long row = trans ? r % rows_pad : r / kblocks;
+  const int kb = (int)(trans ? r / rows_pad : r % kblocks);
+  float a[16];
+  double o[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const long long k = (long long)kb * 16 + i;
-    const long long r = trans ? k : row, cc = trans ? row : k;
-    if (r < m && cc < n) {
-      const float a = (float)load_real(amp, amp_dtype, r * ld_amp + cc);
-      const float2 c = cis_turns(load_real(opd, opd_dtype, r * ld_opd + cc) * iw);
-      v[i] = make_float2(a * c.x, a * c.y);
-    } else {
-      v[i] = make_float2(0.f, 0.f);
-    }
+    const long long sr = trans ? k : row, sc = trans ? row : k;
+    const bool in = sr < m && sc < n;
+    a[i] = in ? (float)load_real(amp, amp_dtype, sr * ld_amp + sc) : 0.f;
+    o[i] = in ? load_real(opd, opd_dtype, sr * ld_opd + sc) : 0.0;
   }
-  char* base = (char*)dst + (long long)w * wave_stride;
-  store_packed16(prec, base, packed_lda(prec, kpad), row, kb, v);
+  const long long lda = packed_lda(prec, kpad);
+  const int w_end = min(nwave, (g + 1) * PH_WAVES);
+  for (int w = g * PH_WAVES; w < w_end; ++w) {
+    const double iw = inv_wl[w];
+    float2 v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float2 c = cis_turns(o[i] * iw);
+      v[i] = make_float2(a[i] * c.x, a[i] * c.y);
+    }
+    store_packed16(prec, (char*)dst + (long long)w * wave_stride, lda, row, kb, v);
+  }
 }
 
 hipError_t launch_phasor(int prec, const void* amp, int amp_dtype, long long ld_amp, const void* opd,
                          int opd_dtype, long long ld_opd, int m, int n, int trans, const double* inv_wl, int nwave,
                          void* dst, long long wave_stride, long long rows_pad, int kpad, hipStream_t s) {
-  const long long total = rows_pad * (kpad / 16) * nwave;
+  const long long total = rows_pad * (kpad / 16) * ((nwave + PH_WAVES - 1) / PH_WAVES);
   if (total <= 0) return hipSuccess;
   const int bs = 256;
   hipLaunchKernelGGL(phasor_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, amp,

@@ -282,3 +282,46 @@ def test_segmented_plan_partial_overlap(L, precision):
         w = L.propagate_dft(w, 5e-6, shape=40, prop_shape=24, oversample=2, precision=precision)
         img = w.insert(img, weight=t_)
     assert rel_l2(img, ref) < TOL_I
+
+
+@pytest.mark.parametrize('tile_n', [64, 128])
+@pytest.mark.parametrize('lambda_group', [1, 3, 8])
+def test_plan_schedules(L, tile_n, lambda_group):
+    """Every schedule knob gives the oracle's image (uneven segment tiles, both
+    coherent paths)."""
+    p, wl, wt, ref = _small_segmented(L, seg_tilts=False)
+    plan = L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, tilts=[L.Tilt(0.3e-6, -0.2e-6)],
+                     precision='bf16x3', lambda_group=lambda_group, tile_n=tile_n)
+    assert plan.info['coherent_fast'] == 1
+    assert rel_l2(plan().double().cpu().numpy(), ref) < TOL_I
+    p, wl, wt, ref = _small_segmented(L, prop_shape=24, seg_tilts=True)
+    plan = L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, prop_shape=24, oversample=2,
+                     tilts=[L.Tilt(0.3e-6, -0.2e-6)], precision='fp32', lambda_group=lambda_group, tile_n=tile_n)
+    assert plan.info['coherent_fast'] == 0
+    assert rel_l2(plan().double().cpu().numpy(), ref) < TOL_I
+
+
+def test_plan_bad_tile_n(L):
+    p, wl, wt, _ = _small_segmented(L, seg_tilts=False)
+    with pytest.raises(ValueError):
+        L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, tile_n=96)
+
+
+def test_autotune(L):
+    """The autotuner times every candidate, keeps one of them, and the tuned
+    plan still matches the oracle."""
+    from lentil_amd import batched, workloads
+    cfg = workloads.mono_256()
+    p = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'], focal_length=10)
+    wl = np.linspace(600e-9, 700e-9, 8)
+    plan = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, autotune=True)
+    assert set(plan.tuning) == set(batched._CANDIDATES)
+    assert all(t > 0 for t in plan.tuning.values())
+    assert (plan.lambda_group, plan.tile_n) == min(plan.tuning, key=plan.tuning.get)
+    ref_p = optics.Pupil(cfg['amp'], cfg['opd'], cfg['dx'], 10)
+    ref = optics.broadband_image(ref_p, wl, np.ones_like(wl), 5e-6, 128, 2)
+    assert rel_l2(plan().double().cpu().numpy(), ref) < TOL_I
+    # a second plan of the same shapes reuses the choice without timing again
+    again = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, autotune=True)
+    assert not hasattr(again, 'tuning')
+    assert (again.lambda_group, again.tile_n) == (plan.lambda_group, plan.tile_n)
