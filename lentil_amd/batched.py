@@ -162,8 +162,8 @@ class PSFPlan:
         self._amp_opd = []
         tiles = []
         for amp, opd in geo['tiles']:
-            ta = torch.from_numpy(np.ascontiguousarray(amp, dtype=np.float32)).to(dev)
-            to = torch.from_numpy(np.ascontiguousarray(opd, dtype=np.float64)).to(dev)
+            ta = torch.from_numpy(np.array(amp, dtype=np.float32, order='C', copy=True)).to(dev)
+            to = torch.from_numpy(np.array(opd, dtype=np.float64, order='C', copy=True)).to(dev)
             self._amp_opd.append((ta, to))
             tiles.append(_lib.MftxTile(ta.data_ptr(), to.data_ptr(), _lib.DT_F32, _lib.DT_F64, ta.stride(0),
                                        to.stride(0), amp.shape[0], amp.shape[1]))

@@ -58,15 +58,51 @@ std::vector<Work> xcd_order(const std::vector<std::vector<Work>>& clusters) {
   return out;
 }
 
-// Works covering a C of rows_tiled x cols for problems [pb, pe), one cluster per tile row.
-void add_tiles(std::vector<std::vector<Work>>& clusters, int pb, int pe, int64_t rows_tiled, int64_t cols, int bn,
+// Works covering rows_tiled C rows x `slots` slots for problems [pb, pe) with
+// bn-column workgroups (bn / 2 slots each), one cluster per tile row.
+void add_tiles(std::vector<std::vector<Work>>& clusters, int pb, int pe, int64_t rows_tiled, int64_t slots, int bn,
                float* partial = nullptr, long long ldp = 0, int prows = 0, int pcols = 0) {
-  const int tr = (int)(rows_tiled / BM), tc = (int)((cols + bn - 1) / bn);
+  const int bs = slots_per_tile(bn);
+  const int tr = (int)(rows_tiled / BM), tc = (int)((slots + bs - 1) / bs);
   for (int r = 0; r < tr; ++r) {
     std::vector<Work> row;
     for (int c = 0; c < tc; ++c) row.push_back(Work{pb, pe, r, c, partial, ldp, prows, pcols});
     clusters.push_back(row);
   }
+}
+
+// Output grid of one MFT stage with N columns, signed twiddle scale alpha and
+// output shift: the slots and pairing centre the GEMM runs on (see Problem)
+// and the fold of its A operand.  Outputs are paired about the PSF core
+// j = N/2 + shift, snapped to the half-pixel grid, so the two outputs of a
+// slot have comparable magnitude and neither inherits the rounding of a much
+// brighter partner; when the core sits within a pixel of the grid centre the
+// centre itself is used (fewest slots), and when pairing would save less than
+// a quarter of the work (core far from the centre or outside the grid) every
+// slot computes one column directly (c = 0).
+struct OutGrid {
+  int slots, cp, cm;
+  double y0, fold;
+};
+
+OutGrid make_grid(double alpha, double shift, int N) {
+  const int h = N / 2;
+  const double core = h + shift;
+  const double mid = 0.5 * (N - 1);
+  double c;
+  if (std::fabs(core - mid) <= 1.0) {
+    c = mid;
+  } else {
+    c = std::nearbyint(2.0 * core) / 2.0;
+    if (!(c >= 0.0 && c <= N - 1) || std::max(N - std::ceil(c), std::floor(c) + 1) > 0.75 * N) c = 0.0;
+  }
+  OutGrid g;
+  g.cm = (int)std::floor(c);
+  g.cp = (int)std::ceil(c);
+  g.y0 = c - g.cm;
+  g.slots = std::max(N - g.cp, g.cm + 1);
+  g.fold = -alpha * (shift + h - c);
+  return g;
 }
 
 void push_chunks(std::vector<double>& cx, int kpad, double x0) {
@@ -100,7 +136,7 @@ struct mftx_plan {
   Problem* probs1 = nullptr;
   Problem* probs2 = nullptr;
   double* chunk_x = nullptr;
-  double* inv_wl = nullptr;
+  PhasorWave* waves = nullptr;  // per phasor wavelength slot: 1/λ and the stage-1 fold
   int nwork1 = 0, nwork2 = 0;
   int epi2 = EPI_INTENSITY;
   struct Phasor {
@@ -282,7 +318,6 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   if (m > (1 << 20) || n > (1 << 20) || M > (1 << 20) || N > (1 << 20))
     return fail(ctx, MFTX_EINVAL, "dimension too large");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  const double sgn = conj ? 1.0 : -1.0;
   const int prec = precision;
   const bool col_first = cost_col_first(m, n, M, N) < cost_row_first(m, n, M, N);
   // stage-1 operand: f (row-first) or f^T (column-first)
@@ -291,8 +326,13 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   const int64_t c2 = col_first ? N : M;                           // stage-2 output columns
   const int64_t rows1 = round_up(r1, BM);
   const int64_t rows2 = round_up(c1, BM);
+  const double sgn = conj ? 1.0 : -1.0;
+  const double alpha1 = sgn * (col_first ? alpha_r : alpha_c), alpha2 = sgn * (col_first ? alpha_c : alpha_r);
+  const OutGrid g1 = make_grid(alpha1, col_first ? shift_r : shift_c, (int)c1);
+  const OutGrid g2 = make_grid(alpha2, col_first ? shift_c : shift_r, (int)c2);
+  const int s1 = g1.slots, s2 = g2.slots;
   // 128-column tiles when the larger stage still gives every CU a workgroup
-  const int64_t wide = std::max((rows1 / BM) * ((c1 + 127) / 128), (rows2 / BM) * ((c2 + 127) / 128));
+  const int64_t wide = std::max((rows1 / BM) * ((s1 + 63) / 64), (rows2 / BM) * ((s2 + 63) / 64));
   const int bn = wide >= 256 ? 128 : 64;
   const int kg = k_granule(bn);
   const int kpad1 = (int)round_up(k1, kg);
@@ -303,39 +343,50 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   if (rc) return rc;
   char* a1 = (char*)ctx->ws;
   char* a2 = a1 + round_up((int64_t)b1, 256);
-  HIPCHK(ctx, launch_pack(prec, f, f_dtype, m, n, ldf, col_first ? 1 : 0, a1, rows1, kpad1, ctx->stream));
+  // twiddle coordinates: rows x = R + or against U - sr; columns x = S + oc
+  // against V - sc; each stage's output shift is folded into its A operand
+  const double xr0 = -centre(m) + offset_r, xc0 = -centre(n) + offset_c;
+  const double fold1 = g1.fold, fold2 = g2.fold;
+  const double x01 = col_first ? xr0 : xc0, x02 = col_first ? xc0 : xr0;
+  HIPCHK(ctx, launch_pack(prec, f, f_dtype, m, n, ldf, col_first ? 1 : 0, fold1, x01, a1, rows1, kpad1,
+                          ctx->stream));
 
-  // twiddle coordinates: rows x = R + or, U - sr;  columns x = S + oc, V - sc
-  const double xr0 = -centre(m) + offset_r, yu0 = -centre(M) - shift_r;
-  const double xc0 = -centre(n) + offset_c, yv0 = -centre(N) - shift_c;
   Problem p1 = {};
   p1.a = a1;
   p1.lda = packed_lda(prec, kpad1);
-  p1.alpha = sgn * (col_first ? alpha_r : alpha_c);
-  p1.y0 = col_first ? yu0 : yv0;
+  p1.alpha = alpha1;
+  p1.y0 = g1.y0;
   p1.rows = (int)r1;
-  p1.cols = (int)c1;
+  p1.cols = s1;
+  p1.ncols = (int)c1;
+  p1.cp = g1.cp;
+  p1.cm = g1.cm;
   p1.kpad = kpad1;
   p1.scale = 1.0f;
   p1.limit = kpad2;
   p1.dst = a2;
   p1.ldd = packed_lda(prec, kpad2);
-  rc = run_single(ctx, prec, EPI_STORE_PACKED_T, bn, p1, col_first ? xr0 : xc0, rows1);
+  p1.fold = fold2;
+  p1.fold_x0 = x02;
+  rc = run_single(ctx, prec, EPI_STORE_PACKED_T, bn, p1, x01, rows1);
   if (rc) return rc;
 
   Problem p2 = {};
   p2.a = a2;
   p2.lda = packed_lda(prec, kpad2);
-  p2.alpha = sgn * (col_first ? alpha_c : alpha_r);
-  p2.y0 = col_first ? yv0 : yu0;
+  p2.alpha = alpha2;
+  p2.y0 = g2.y0;
   p2.rows = (int)c1;
-  p2.cols = (int)c2;
+  p2.cols = s2;
+  p2.ncols = (int)c2;
+  p2.cp = g2.cp;
+  p2.cm = g2.cm;
   p2.kpad = kpad2;
   p2.scale = (float)scale;
   p2.dst = F;
   p2.ldd = ldF;
   p2.trans = col_first ? 0 : 1;  // C is F (column-first) or F^T (row-first)
-  return run_single(ctx, prec, EPI_STORE_CPLX, bn, p2, col_first ? xc0 : xr0, rows2);
+  return run_single(ctx, prec, EPI_STORE_CPLX, bn, p2, x02, rows2);
 }
 
 int mftx_phasor(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd, int opd_dtype,
@@ -528,7 +579,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     for (int i : c.members) tile_waves[{problems[i].tile, c.col_first ? 1 : 0}].push_back(problems[i].wave);
   struct TileOp { int64_t rows; int kpad; size_t off; std::map<int, int> slot; int wave_off; std::vector<int> waves; };
   std::map<std::pair<int, int>, TileOp> tops;
-  std::vector<int> inv_order;
+  std::vector<std::tuple<int, int, int>> wave_slots;  // (tile, orientation, wave) per phasor slot
   for (auto& kv : tile_waves) {
     auto v = kv.second;
     std::sort(v.begin(), v.end());
@@ -538,11 +589,11 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     TileOp op;
     op.rows = round_up(tr ? T.n : T.m, BM);
     op.kpad = (int)round_up(tr ? T.m : T.n, kg);
-    op.wave_off = (int)inv_order.size();
+    op.wave_off = (int)wave_slots.size();
     op.waves = v;
     for (size_t j = 0; j < v.size(); ++j) {
       op.slot[v[j]] = (int)j;
-      inv_order.push_back(v[j]);
+      wave_slots.emplace_back(kv.first.first, kv.first.second, v[j]);
     }
     op.off = lay.take(packed_bytes(prec, op.rows, op.kpad) * v.size());
     tops[kv.first] = op;
@@ -563,6 +614,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
 
   // ---- descriptors ---------------------------------------------------------------
   std::vector<Problem> p1, p2;
+  std::map<std::tuple<int, int, int>, std::pair<double, double>> stage1_fold;  // (tile, orientation, wave) -> fold, x0
   std::vector<double> cx;
   std::vector<std::vector<Work>> clus1, clus2;
   struct Patch { int idx; size_t a, dst; };  // workspace offsets to rebase after allocation
@@ -577,18 +629,30 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       const mftx_tile& T = tiles[P.tile];
       const TileOp& op = tops[{P.tile, cf ? 1 : 0}];
       const int kp2 = (int)round_up(cf ? T.n : T.m, kg);
+      const int n1 = cf ? K.M : K.N, n2 = cf ? K.N : K.M;  // output columns of stages 1 and 2
+      const double x01 = cf ? -centre(T.m) + P.offset_r : -centre(T.n) + P.offset_c;
+      const double x02 = cf ? -centre(T.n) + P.offset_c : -centre(T.m) + P.offset_r;
       Problem q = {};
       q.lda = packed_lda(prec, op.kpad);
       q.alpha = -(cf ? P.alpha_r : P.alpha_c);
-      q.y0 = cf ? -centre(K.M) - K.sr : -centre(K.N) - K.sc;
+      const OutGrid g1 = make_grid(q.alpha, cf ? K.sr : K.sc, n1);
+      const OutGrid g2 = make_grid(-(cf ? K.ac : K.ar), cf ? K.sc : K.sr, n2);
+      q.y0 = g1.y0;
       q.rows = (int)(cf ? T.n : T.m);
-      q.cols = cf ? K.M : K.N;
+      q.cols = g1.slots;
+      q.ncols = n1;
+      q.cp = g1.cp;
+      q.cm = g1.cm;
       q.kpad = op.kpad;
       q.chunk_begin = (int)cx.size();
-      push_chunks(cx, op.kpad, cf ? -centre(T.m) + P.offset_r : -centre(T.n) + P.offset_c);
+      push_chunks(cx, op.kpad, x01);
       q.scale = 1.0f;
       q.limit = kp2;
       q.ldd = packed_lda(prec, c.ktot);
+      // the stage-2 fold for this member's K segment; the stage-1 fold goes to the phasor
+      q.fold = g2.fold;
+      q.fold_x0 = x02;
+      stage1_fold[{P.tile, cf ? 1 : 0, P.wave}] = {g1.fold, x01};
       const size_t elem = prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(uint16_t);
       patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)op.slot.at(P.wave) * packed_bytes(prec, op.rows, op.kpad),
                              c.abuf + (size_t)c.kbeg[j] * elem});
@@ -597,11 +661,16 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     }
     // stage 2, one problem per class (K concatenated over members)
     Problem q = {};
+    const int n2 = cf ? K.N : K.M;
     q.lda = packed_lda(prec, c.ktot);
     q.alpha = -(cf ? K.ac : K.ar);
-    q.y0 = cf ? -centre(K.N) - K.sc : -centre(K.M) - K.sr;
+    const OutGrid g2 = make_grid(q.alpha, cf ? K.sc : K.sr, n2);
+    q.y0 = g2.y0;
     q.rows = cf ? K.M : K.N;
-    q.cols = cf ? K.N : K.M;
+    q.cols = g2.slots;
+    q.ncols = n2;
+    q.cp = g2.cp;
+    q.cm = g2.cm;
     q.kpad = c.ktot;
     q.chunk_begin = (int)cx.size();
     for (size_t j = 0; j < c.members.size(); ++j) {
@@ -664,7 +733,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
         const ClassInfo& c0 = cls[gi.lgroups[g][0]];
         const size_t poff = gi.partial + sizeof(float) * gi.prow * gi.pcol * g;
         // the partial image offset rides in the pointer field until the workspace exists
-        add_tiles(clus2, pb, (int)p2_launch.size(), c0.rows2, c0.col_first ? c0.key.N : c0.key.M, tile_n,
+        add_tiles(clus2, pb, (int)p2_launch.size(), c0.rows2, p2[c0.p2].cols, tile_n,
                   (float*)(uintptr_t)poff, (long long)gi.pcol, c0.key.M, c0.key.N);
       }
     }
@@ -673,7 +742,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       const int pb = (int)p2_launch.size();
       p2_launch.push_back(p2[c.p2]);
       patch2_launch.push_back(patch2[c.p2]);
-      add_tiles(clus2, pb, pb + 1, c.rows2, c.col_first ? c.key.N : c.key.M, tile_n);
+      add_tiles(clus2, pb, pb + 1, c.rows2, p2[c.p2].cols, tile_n);
     }
   }
   std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
@@ -683,7 +752,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   const size_t o_p1 = lay.take(sizeof(Problem) * std::max<size_t>(p1.size(), 1));
   const size_t o_p2 = lay.take(sizeof(Problem) * std::max<size_t>(p2_launch.size(), 1));
   const size_t o_cx = lay.take(sizeof(double) * std::max<size_t>(cx.size(), 1));
-  const size_t o_iw = lay.take(sizeof(double) * std::max<size_t>(inv_order.size(), 1));
+  const size_t o_iw = lay.take(sizeof(PhasorWave) * std::max<size_t>(wave_slots.size(), 1));
   std::vector<size_t> fb_off;
   if (!fast)
     for (size_t g = 0; g < groups.size(); ++g) fb_off.push_back(lay.take(sizeof(float2) * image_rows * image_cols));
@@ -703,15 +772,18 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   }
   for (auto& w : w2)
     if (w.partial) w.partial = (float*)(base + (size_t)(uintptr_t)w.partial);
-  std::vector<double> iw(inv_order.size());
-  for (size_t i = 0; i < inv_order.size(); ++i) iw[i] = 1.0 / wavelengths[inv_order[i]];
+  std::vector<PhasorWave> iw(wave_slots.size());
+  for (size_t i = 0; i < wave_slots.size(); ++i) {
+    const auto& fx = stage1_fold.at(wave_slots[i]);
+    iw[i] = PhasorWave{1.0 / wavelengths[std::get<2>(wave_slots[i])], fx.first, fx.second};
+  }
 
   plan->works1 = (Work*)(base + o_w1);
   plan->works2 = (Work*)(base + o_w2);
   plan->probs1 = (Problem*)(base + o_p1);
   plan->probs2 = (Problem*)(base + o_p2);
   plan->chunk_x = (double*)(base + o_cx);
-  plan->inv_wl = (double*)(base + o_iw);
+  plan->waves = (PhasorWave*)(base + o_iw);
   plan->nwork1 = (int)w1.size();
   plan->nwork2 = (int)w2.size();
   hipError_t e = hipSuccess;
@@ -723,7 +795,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   up(plan->probs1, p1.data(), sizeof(Problem) * p1.size());
   up(plan->probs2, p2_launch.data(), sizeof(Problem) * p2_launch.size());
   up(plan->chunk_x, cx.data(), sizeof(double) * cx.size());
-  up(plan->inv_wl, iw.data(), sizeof(double) * iw.size());
+  up(plan->waves, iw.data(), sizeof(PhasorWave) * iw.size());
   if (e != hipSuccess) {
     (void)hipFree(plan->ws);
     delete plan;
@@ -809,7 +881,7 @@ static hipError_t launch_phasors(const mftx_plan* plan, hipStream_t s) {
   for (auto& ph : plan->phasors) {
     hipError_t e = launch_phasor(plan->prec, ph.tile.amp, ph.tile.amp_dtype, ph.tile.ld_amp, ph.tile.opd,
                                  ph.tile.opd_dtype, ph.tile.ld_opd, (int)ph.tile.m, (int)ph.tile.n, ph.trans,
-                                 plan->inv_wl + ph.wave_off, ph.nwave, ph.dst, ph.wave_stride, ph.rows_pad, ph.kpad, s);
+                                 plan->waves + ph.wave_off, ph.nwave, ph.dst, ph.wave_stride, ph.rows_pad, ph.kpad, s);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -1,58 +1,66 @@
-// Complex GEMM C = A . B of the two MFT stages on gfx950 MFMA.
+// Complex x twiddle GEMM of the two MFT stages on gfx950 MFMA.
 //
-// A is data in HBM (packed, see mftx_internal.h); B is a twiddle matrix
-// B[k][j] = cis(alpha * x_k * y_j) generated on the fly into LDS:
+// A is data in HBM (packed, see mftx_internal.h); the twiddles
+// cis(alpha * x_k * Y_s) are generated on the fly into LDS:
 //     stage 1:  G   = f   . E2      (stored transposed, packed, as G^T)
 //     stage 2:  F^T = G^T . E1^T    (|F|^2 accumulated, or F stored)
 // which is the triple product of lentil fourier.py:97 with the twiddles of
 // fourier.py:108-109.
 //
-// Workgroup: 2 x WN waves, each wave a 64 x 32 block of C as two MFMA 32x32
-// tiles; C tile BM x BN = 128 x 32 WN.  K runs in chunks of BKC = 16 KS staged
-// in double-buffered LDS, one barrier per chunk.  Two configurations:
-//   WN = 2, KS = 1: 64-column tiles, 48 KiB LDS, two workgroups per CU;
-//   WN = 4, KS = 2: 128-column tiles, 32-wide chunks, 128 KiB LDS, one
-//                   workgroup (8 waves) per CU.
+// Centred output grid.  With the shift folded into A by its producer, each
+// stage is C[i][j] = sum_k A'[i][k] cis(alpha x_k Y_j) with Y symmetric about
+// 0, so one GEMM over the slots Y_s >= 0 with the real twiddle matrices
+// cos and sin gives P = A' cos, Q = A' sin and both output columns
+// C(+Y_s) = P + iQ and C(-Y_s) = P - iQ: half the MFMAs and half the
+// twiddles of the direct complex product.
+//
+// Workgroup: 4 x WN waves, each wave a 32-row x 32-slot block (two output
+// columns per slot) with accumulators P_re, P_im, Q_re, Q_im; C tile 128 rows
+// x 32 WN slots.  K runs in chunks of 16 KS staged in double-buffered LDS, one
+// barrier per chunk.  Two configurations:
+//   WN = 1, KS = 1: 32 slots (64 output columns), 40 KiB LDS, two workgroups per CU;
+//   WN = 2, KS = 2: 64 slots (128 output columns), 32-wide K chunks, 96 KiB LDS,
+//                   one workgroup (8 waves) per CU.
 // One chunk ahead, each thread stages A items (32 bytes of one row: a BF16
 // plane or four complex64 values; global -> registers -> LDS) and generates
-// 4 KS consecutive twiddles of one B column, t_i = P * Q_i with
-// P = cis(alpha * x * y) (FP64 argument and range reduction, FP32 sin/cos)
-// per chunk and the per-problem table Q_i = cis(alpha * i * y).  The twiddle
-// VALU work is interleaved with the MFMAs of the current chunk
+// 2 KS consecutive twiddles of one slot, t_i = P * Q_i with
+// P = cis(alpha * x * Y) (FP64 argument and range reduction, FP32 sin/cos) per
+// chunk and the per-problem table Q_i = cis(alpha * i * Y).  The twiddle VALU
+// work is interleaved with the MFMAs of the current chunk
 // (sched_group_barrier), and the loop body is one basic block.
 //
 // Precision forms
-//   PREC_FP32   : complex64 operands, v_mfma_f32_32x32x2_f32, 4 real MFMAs per
-//                 complex product.
+//   PREC_FP32   : complex64 A, v_mfma_f32_32x32x2_f32: 4 real MFMAs per
+//                 (A element, twiddle) pair.
 //   PREC_BF16X3 : every FP32 operand x is split x = hi + lo (two BF16, round
 //                 to nearest even); a real product is hi*hi + hi*lo + lo*hi on
 //                 v_mfma_f32_32x32x16_bf16 with FP32 accumulation, 12 MFMAs per
-//                 complex product.
+//                 32x32 slot block and 16-wide K step.
 #include "mftx_device.h"
 
 namespace mftx {
 
 // Geometry of one tile configuration.
 template <int WN, int KS> struct Cfg {
-  static constexpr int BN = 32 * WN;             // C cols per workgroup
-  static constexpr int BKC = 16 * KS;            // K per chunk
-  static constexpr int NT = 64 * WAVES_M * WN;   // threads per workgroup
-  static constexpr int A_ITEMS = 512 * KS / NT;  // A staging items per thread
-  static constexpr int TW = 4 * KS;              // twiddles per thread per chunk
-  static constexpr int WG_PER_CU = WN <= 2 ? 2 : 1;
+  static constexpr int BS = 32 * WN;               // slots per workgroup
+  static constexpr int BKC = 16 * KS;              // K per chunk
+  static constexpr int NT = 64 * WAVES_M * WN;     // threads per workgroup
+  static constexpr int A_ITEMS = 512 * KS / NT;    // A staging items per thread
+  static constexpr int TW = 2 * KS;                // twiddles per thread per chunk
+  static constexpr int WG_PER_CU = WN == 1 ? 2 : 1;
   static_assert((512 * KS) % NT == 0, "A items must divide evenly");
-  static_assert(BN * BKC == NT * TW, "one twiddle item per thread");
+  static_assert(BS * BKC == NT * TW, "one twiddle item per thread");
 };
 
 template <int PREC, int WN, int KS> struct Lds;
 
-// bf16x3: 4 planes (re_hi, re_lo, im_hi, im_lo) of [row][BKC] bf16, made of
-// 16-byte units of 8 K values.  Units are swizzled per row,
-// unit' = unit ^ swz_key(row), so the 16-byte accesses of 16 consecutive
-// lanes (16 consecutive rows, one unit) cover all 64 banks once.
+// bf16x3: 4 planes of [row][16 KS] bf16 (A: re_hi, re_lo, im_hi, im_lo; B:
+// cos_hi, cos_lo, sin_hi, sin_lo), made of 16-byte units of 8 K values.  Units
+// are swizzled per row, unit' = unit ^ swz_key(row), so the 16-byte accesses of
+// 16 consecutive lanes (16 consecutive rows, one unit) cover all 64 banks once.
 template <int WN, int KS> struct Lds<PREC_BF16X3, WN, KS> {
   __bf16 a[2][4][BM][16 * KS];
-  __bf16 b[2][4][Cfg<WN, KS>::BN][16 * KS];
+  __bf16 b[2][4][Cfg<WN, KS>::BS][16 * KS];
 };
 template <int KS> __device__ __forceinline__ int swz_key(int row) {
   return KS == 1 ? (row >> 3) & 1 : (row >> 2) & 3;
@@ -60,12 +68,13 @@ template <int KS> __device__ __forceinline__ int swz_key(int row) {
 // element offset of logical unit u of a row
 template <int KS> __device__ __forceinline__ int swz(int row, int u) { return 8 * (u ^ swz_key<KS>(row)); }
 
-// fp32: complex64 [row][BKC + 2]; the padded row stride keeps the 16-byte
-// stores of 16 consecutive rows and the 8-byte fragment reads on distinct banks.
+// fp32: complex64 [row][BKC + 2] (B: (cos, sin) pairs); the padded row stride
+// keeps the 16-byte stores of 16 consecutive rows and the 8-byte fragment
+// reads on distinct banks.
 template <int WN, int KS> struct Lds<PREC_FP32, WN, KS> {
   static constexpr int RS = 16 * KS + 2;
   float2 a[2][BM][RS];
-  float2 b[2][Cfg<WN, KS>::BN][RS];
+  float2 b[2][Cfg<WN, KS>::BS][RS];
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -77,16 +86,17 @@ typedef __attribute__((address_space(1))) const u32x4 gu32x4;  // global-memory 
 // is row (i & 15) + 16 ((i >> 6) & 7), part (i >> 4) & 3, K step i >> 9, so
 // 16 consecutive lanes store 16 consecutive rows of one part (conflict-free
 // LDS stores).  Thread t stages items t + j * NT.
-// B generation: BN x 4 items (column, group of TW K values), one per thread:
-// column (t & 15) + 16 (t >> 6), group (t >> 4) & 3.
+// B generation: BS x 8 items (slot, group of TW K values), one per thread:
+// slot (t & 15) + 16 (t >> 7), group (t >> 4) & 7.
 __device__ __forceinline__ int item_row(int i) { return (i & 15) + 16 * ((i >> 6) & 7); }
 __device__ __forceinline__ int item_part(int i) { return (i >> 4) & 3; }
 __device__ __forceinline__ int item_kstep(int i) { return i >> 9; }
-__device__ __forceinline__ int bitem_col(int t) { return (t & 15) + 16 * (t >> 6); }
+__device__ __forceinline__ int bitem_slot(int t) { return (t & 15) + 16 * (t >> 7); }
+__device__ __forceinline__ int bitem_group(int t) { return (t >> 4) & 7; }
 
 // Staged chunk in registers: A prefetch and the generated twiddles.
 // bf16x3: b[p] holds plane p of the TW K values (TW / 2 words); fp32: the TW
-// complex values, 2 per 16 bytes.  Native vector types keep these in VGPRs.
+// (cos, sin) pairs, two per 16 bytes.  Native vector types keep these in VGPRs.
 template <int WN, int KS> struct Staged {
   u32x4 a[Cfg<WN, KS>::A_ITEMS][2];
   u32x4 b[4];
@@ -97,17 +107,17 @@ struct StageProb {
   const char* a;
   long long lda;
   double alpha4;   // 4 * alpha: twiddle arguments in quarter turns
-  double y;        // y of this thread's B column
+  double y;        // Y of this thread's slot
   int nch;         // chunks in the problem
   int cb;          // chunk_begin (index of the problem's first 16-wide K block in chunk_x)
 };
 
 template <int KS, int NQ>
-__device__ __forceinline__ void set_stage(const Problem& P, int col, StageProb& sp, float2 (&q)[NQ]) {
+__device__ __forceinline__ void set_stage(const Problem& P, int slot, StageProb& sp, float2 (&q)[NQ]) {
   sp.a = (const char*)P.a;
   sp.lda = P.lda;
   sp.alpha4 = 4.0 * P.alpha;
-  sp.y = P.y0 + (double)col;
+  sp.y = P.y0 + (double)slot;
   sp.nch = P.kpad / (16 * KS);
   sp.cb = P.chunk_begin;
 #pragma unroll
@@ -118,7 +128,7 @@ __device__ __forceinline__ void set_stage(const Problem& P, int col, StageProb& 
 // 16-wide K block).
 template <int KS>
 __device__ __forceinline__ double twiddle_x(const double* __restrict__ chunk_x, const StageProb& sp, int c, int g) {
-  const int k = 4 * KS * g;
+  const int k = 2 * KS * g;
   return chunk_x[sp.cb + KS * c + (k >> 4)] + (double)(k & 15);
 }
 
@@ -140,26 +150,26 @@ __device__ __forceinline__ void load_a(const StageProb& sp, long long row0, int 
   }
 }
 
-// TW consecutive twiddles from K origin x for this thread's column.
+// TW consecutive twiddles from K origin x for this thread's slot.
 template <int PREC, int WN, int KS>
-__device__ __forceinline__ void gen_b(double x, const StageProb& sp, const float2 (&q)[4 * KS],
+__device__ __forceinline__ void gen_b(double x, const StageProb& sp, const float2 (&q)[2 * KS],
                                       Staged<WN, KS>& st) {
-  constexpr int TW = 4 * KS;
+  constexpr int TW = 2 * KS;
   const float2 pf = cis_quarter(sp.alpha4 * (x * sp.y));
   float2 t[TW];
 #pragma unroll
   for (int i = 0; i < TW; ++i) t[i] = cmul(pf, q[i]);
   if (PREC == PREC_BF16X3) {
-    uint32_t rh[4] = {}, rl[4] = {}, ih[4] = {}, il[4] = {};
+    uint32_t ch[2] = {}, cl[2] = {}, sh[2] = {}, sl[2] = {};
 #pragma unroll
     for (int i = 0; i < TW / 2; ++i) {
-      split2(t[2 * i].x, t[2 * i + 1].x, rh[i], rl[i]);
-      split2(t[2 * i].y, t[2 * i + 1].y, ih[i], il[i]);
+      split2(t[2 * i].x, t[2 * i + 1].x, ch[i], cl[i]);
+      split2(t[2 * i].y, t[2 * i + 1].y, sh[i], sl[i]);
     }
-    st.b[0] = u32x4{rh[0], rh[1], rh[2], rh[3]};
-    st.b[1] = u32x4{rl[0], rl[1], rl[2], rl[3]};
-    st.b[2] = u32x4{ih[0], ih[1], ih[2], ih[3]};
-    st.b[3] = u32x4{il[0], il[1], il[2], il[3]};
+    st.b[0] = u32x4{ch[0], ch[1], 0u, 0u};
+    st.b[1] = u32x4{cl[0], cl[1], 0u, 0u};
+    st.b[2] = u32x4{sh[0], sh[1], 0u, 0u};
+    st.b[3] = u32x4{sl[0], sl[1], 0u, 0u};
   } else {
     auto bits = [](float v) { return __float_as_uint(v); };
 #pragma unroll
@@ -169,23 +179,24 @@ __device__ __forceinline__ void gen_b(double x, const StageProb& sp, const float
 }
 
 template <int PREC, int WN, int KS>
-__device__ __forceinline__ void write_b(Lds<PREC, WN, KS>& L, int buf, int col, int g, const Staged<WN, KS>& st) {
+__device__ __forceinline__ void write_b(Lds<PREC, WN, KS>& L, int buf, int slot, int g, const Staged<WN, KS>& st) {
+  constexpr int TW = 2 * KS;
   if (PREC == PREC_BF16X3) {
     auto& LB = reinterpret_cast<Lds<PREC_BF16X3, WN, KS>&>(L);
-    if (KS == 1) {  // 4 K values: half of unit g >> 1
-      const int k = swz<KS>(col, g >> 1) + 4 * (g & 1);
+    const int k = TW * g;  // first K of the group: unit k / 8, offset k % 8
+    const int e = swz<KS>(slot, k >> 3) + (k & 7);
+    if (KS == 1) {
 #pragma unroll
-      for (int p = 0; p < 4; ++p) *(u32x2*)&LB.b[buf][p][col][k] = u32x2{st.b[p][0], st.b[p][1]};
-    } else {        // 8 K values: unit g
-      const int k = swz<KS>(col, g);
+      for (int p = 0; p < 4; ++p) *(uint32_t*)&LB.b[buf][p][slot][e] = st.b[p][0];
+    } else {
 #pragma unroll
-      for (int p = 0; p < 4; ++p) *(u32x4*)&LB.b[buf][p][col][k] = st.b[p];
+      for (int p = 0; p < 4; ++p) *(u32x2*)&LB.b[buf][p][slot][e] = u32x2{st.b[p][0], st.b[p][1]};
     }
   } else {
     auto& LF = reinterpret_cast<Lds<PREC_FP32, WN, KS>&>(L);
-    u32x4* b = (u32x4*)&LF.b[buf][col][g * 4 * KS];
+    u32x4* b = (u32x4*)&LF.b[buf][slot][g * TW];
 #pragma unroll
-    for (int i = 0; i < 2 * KS; ++i) b[i] = st.b[i];
+    for (int i = 0; i < TW / 2; ++i) b[i] = st.b[i];
   }
 }
 
@@ -212,105 +223,135 @@ __device__ __forceinline__ void write_a(Lds<PREC, WN, KS>& L, int buf, int tid, 
 #define SGB(mask, n) __builtin_amdgcn_sched_group_barrier((mask), (n), 0)
 constexpr int SG_VALU = 0x002, SG_MFMA = 0x008, SG_VMEM_RD = 0x020, SG_DSR = 0x100;
 
-__device__ __forceinline__ bf16x8_t neg_bf16x8(bf16x8_t v) {
-  return __builtin_bit_cast(bf16x8_t, __builtin_bit_cast(u32x4, v) ^ 0x80008000u);
+// One 32x32 slot block, K = 16: P += A cos, Q += A sin for A = (re, im), each
+// real product as hi*hi + hi*lo + lo*hi.  a: planes (re_hi, re_lo, im_hi,
+// im_lo); b: planes (cos_hi, cos_lo, sin_hi, sin_lo).  acc: P_re, P_im, Q_re, Q_im.
+__device__ __forceinline__ void sblock_bf16(const bf16x8_t (&a)[4], const bf16x8_t (&b)[4], f32x16 (&acc)[4]) {
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int ia = t == 2 ? 1 : 0;   // hi, hi, lo  of A
+    const int ib = t == 1 ? 1 : 0;   // hi, lo, hi  of the twiddle
+    acc[0] = MFMA_BF16(a[ia], b[ib], acc[0]);
+    acc[1] = MFMA_BF16(a[2 + ia], b[ib], acc[1]);
+    acc[2] = MFMA_BF16(a[ia], b[2 + ib], acc[2]);
+    acc[3] = MFMA_BF16(a[2 + ia], b[2 + ib], acc[3]);
+  }
 }
 
-// One complex 32x32 block, K = 16: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br,
-// each real product as hi*hi + hi*lo + lo*hi.  a/b: planes (rh, rl, ih, il);
-// nbi: (-Bi_hi, -Bi_lo).
-__device__ __forceinline__ void cblock_bf16(const bf16x8_t (&a)[4], const bf16x8_t (&b)[4], const bf16x8_t (&nbi)[2],
-                                            f32x16& r, f32x16& m) {
-  r = MFMA_BF16(a[0], b[0], r);
-  m = MFMA_BF16(a[0], b[2], m);
-  r = MFMA_BF16(a[0], b[1], r);
-  m = MFMA_BF16(a[0], b[3], m);
-  r = MFMA_BF16(a[1], b[0], r);
-  m = MFMA_BF16(a[1], b[2], m);
-  r = MFMA_BF16(a[2], nbi[0], r);
-  m = MFMA_BF16(a[2], b[0], m);
-  r = MFMA_BF16(a[2], nbi[1], r);
-  m = MFMA_BF16(a[2], b[1], m);
-  r = MFMA_BF16(a[3], nbi[0], r);
-  m = MFMA_BF16(a[3], b[0], m);
-}
-
-// End of one problem: intensity accumulates w * |C|^2 in registers; the
-// store forms write C * scale (complex, or packed transposed as the next
-// stage's A operand).  Accumulator element e of a lane's 32x32 tile i is C
-// row 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32.
+// End of one problem.  Accumulator element e of a lane is C row
+// 8 (e / 4) + 4 (lane / 32) + e % 4 of its 32-row block and slot lane % 32;
+// the slot's two output columns are jp = cp + s (P + iQ) and jm = cm - s
+// (P - iQ).  Intensity accumulates w |C|^2 for both in registers; the store
+// forms write C * scale (complex, or packed transposed as the next stage's A',
+// times the next stage's fold).
 template <int PREC, int EPI>
-__device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& wk, int BN, int wr, int wc, int lane,
-                                                 const f32x16 (&cr)[2], const f32x16 (&ci)[2], f32x16 (&iacc)[2]) {
+__device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& wk, int BS, int wr, int wc, int lane,
+                                                 const f32x16 (&acc)[4], f32x16 (&ip), f32x16 (&im)) {
+  const int s = wk.tc * BS + wc * 32 + (lane & 31);
+  const int N = P.ncols;
+  const int jp = P.cp + s, jm = P.cm - s;
   if (EPI == EPI_INTENSITY) {
     const float w = P.scale;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int e = 0; e < 16; ++e) {
+      const float pr = acc[0][e], pi = acc[1][e], qr = acc[2][e], qi = acc[3][e];
+      const float ar = pr - qi, ai = pi + qr;   // P + iQ
+      const float br = pr + qi, bi = pi - qr;   // P - iQ
+      ip[e] = __builtin_fmaf(w, __builtin_fmaf(ar, ar, ai * ai), ip[e]);
+      im[e] = __builtin_fmaf(w, __builtin_fmaf(br, br, bi * bi), im[e]);
+    }
+    return;
+  }
+  const float sc = P.scale;
+  const bool has_p = jp < N, has_m = jm >= 0 && jm < N && jm != jp;
 #pragma unroll
-      for (int e = 0; e < 16; ++e)
-        iacc[i][e] = __builtin_fmaf(w, __builtin_fmaf(cr[i][e], cr[i][e], ci[i][e] * ci[i][e]), iacc[i][e]);
-  } else {
-    const float sc = P.scale;
-    const int col = wk.tc * BN + wc * 32 + (lane & 31);
+  for (int g = 0; g < 4; ++g) {
+    // rows row0 .. row0+3 of C in this lane
+    const int row0 = wk.tr * BM + wr * 32 + 8 * g + 4 * (lane >> 5);
+    float pre[4], pim[4], mre[4], mim[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int e = 0; e < 4; ++e) {
+      const float pr = acc[0][4 * g + e], pi = acc[1][4 * g + e], qr = acc[2][4 * g + e], qi = acc[3][4 * g + e];
+      pre[e] = (pr - qi) * sc;
+      pim[e] = (pi + qr) * sc;
+      mre[e] = (pr + qi) * sc;
+      mim[e] = (pi - qr) * sc;
+    }
+    if (EPI == EPI_STORE_CPLX) {
+      float2* dst = (float2*)P.dst;
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        // rows row0 .. row0+3 of C in this lane (32x32 accumulator layout)
-        const int row0 = wk.tr * BM + wr * 64 + i * 32 + 8 * g + 4 * (lane >> 5);
-        float re[4], im[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          re[e] = cr[i][4 * g + e] * sc;
-          im[e] = ci[i][4 * g + e] * sc;
+      for (int e = 0; e < 4; ++e) {
+        const int row = row0 + e;
+        if (row < P.rows) {
+          if (has_p) {
+            const float2 v = make_float2(pre[e], pim[e]);
+            if (P.trans) dst[(long long)jp * P.ldd + row] = v;
+            else dst[(long long)row * P.ldd + jp] = v;
+          }
+          if (has_m) {
+            const float2 v = make_float2(mre[e], mim[e]);
+            if (P.trans) dst[(long long)jm * P.ldd + row] = v;
+            else dst[(long long)row * P.ldd + jm] = v;
+          }
         }
-        if (EPI == EPI_STORE_CPLX) {
-          float2* dst = (float2*)P.dst;
+      }
+    } else if (row0 < P.limit) {  // EPI_STORE_PACKED_T: destination row = output column, K = C row
+      // the next stage's fold along its K index (this C's row)
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = row0 + e;
-            if (row < P.rows && col < P.cols) {
-              const float2 v = make_float2(re[e], im[e]);
-              if (P.trans) dst[(long long)col * P.ldd + row] = v;
-              else dst[(long long)row * P.ldd + col] = v;
-            }
-          }
-        } else if (row0 < P.limit) {  // EPI_STORE_PACKED_T: destination row = C col, K = C row
-          if (PREC == PREC_BF16X3) {
-            uint32_t rh0, rl0, rh1, rl1, ih0, il0, ih1, il1;
-            split2(re[0], re[1], rh0, rl0);
-            split2(re[2], re[3], rh1, rl1);
-            split2(im[0], im[1], ih0, il0);
-            split2(im[2], im[3], ih1, il1);
-            __bf16* d = (__bf16*)P.dst + (long long)col * P.ldd + (long long)(row0 >> 4) * 64 + (row0 & 15);
-            *(uint2*)(d + 0) = make_uint2(rh0, rh1);
-            *(uint2*)(d + 16) = make_uint2(rl0, rl1);
-            *(uint2*)(d + 32) = make_uint2(ih0, ih1);
-            *(uint2*)(d + 48) = make_uint2(il0, il1);
-          } else {
-            float4* d = (float4*)((float2*)P.dst + (long long)col * P.ldd + row0);
-            d[0] = make_float4(re[0], im[0], re[1], im[1]);
-            d[1] = make_float4(re[2], im[2], re[3], im[3]);
-          }
+      for (int e = 0; e < 4; ++e) {
+        const float2 f = cis_turns(P.fold * (P.fold_x0 + (double)(row0 + e)));
+        float t = pre[e] * f.x - pim[e] * f.y;
+        pim[e] = pre[e] * f.y + pim[e] * f.x;
+        pre[e] = t;
+        t = mre[e] * f.x - mim[e] * f.y;
+        mim[e] = mre[e] * f.y + mim[e] * f.x;
+        mre[e] = t;
+      }
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const int j = side ? jm : jp;
+        if (side ? !has_m : !has_p) continue;
+        const float* re = side ? mre : pre;
+        const float* imv = side ? mim : pim;
+        if (PREC == PREC_BF16X3) {
+          uint32_t rh0, rl0, rh1, rl1, ih0, il0, ih1, il1;
+          split2(re[0], re[1], rh0, rl0);
+          split2(re[2], re[3], rh1, rl1);
+          split2(imv[0], imv[1], ih0, il0);
+          split2(imv[2], imv[3], ih1, il1);
+          __bf16* d = (__bf16*)P.dst + (long long)j * P.ldd + (long long)(row0 >> 4) * 64 + (row0 & 15);
+          *(uint2*)(d + 0) = make_uint2(rh0, rh1);
+          *(uint2*)(d + 16) = make_uint2(rl0, rl1);
+          *(uint2*)(d + 32) = make_uint2(ih0, ih1);
+          *(uint2*)(d + 48) = make_uint2(il0, il1);
+        } else {
+          float4* d = (float4*)((float2*)P.dst + (long long)j * P.ldd + row0);
+          d[0] = make_float4(re[0], imv[0], re[1], imv[1]);
+          d[1] = make_float4(re[2], imv[2], re[3], imv[3]);
         }
       }
     }
   }
 }
 
-// The workgroup's summed intensity tile into its partial image (transposed
-// like the problems' trans flag), clipped to the partial extent.
-__device__ __forceinline__ void store_partial(const Work& wk, int trans, int BN, int wr, int wc, int lane,
-                                              const f32x16 (&iacc)[2]) {
-  const int col = wk.tc * BN + wc * 32 + (lane & 31);
+// The workgroup's summed intensity into its partial image (transposed like
+// the problems' trans flag), clipped to the partial extent.
+__device__ __forceinline__ void store_partial(const Work& wk, const Problem& P, int BS, int wr, int wc, int lane,
+                                              const f32x16& ip, const f32x16& im) {
+  const int s = wk.tc * BS + wc * 32 + (lane & 31);
+  const int jp = P.cp + s, jm = P.cm - s;
+  const bool has_p = jp < P.ncols, has_m = jm >= 0 && jm < P.ncols && jm != jp;
+  const int trans = P.trans;
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int row = wk.tr * BM + wr * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-      const int pr = trans ? col : row;
-      const int pcn = trans ? row : col;
-      if (pr < wk.prows && pcn < wk.pcols) wk.partial[(long long)pr * wk.ldp + pcn] = iacc[i][e];
+  for (int e = 0; e < 16; ++e) {
+    const int row = wk.tr * BM + wr * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+    if (has_p) {
+      const int pr = trans ? jp : row, pc = trans ? row : jp;
+      if (pr < wk.prows && pc < wk.pcols) wk.partial[(long long)pr * wk.ldp + pc] = ip[e];
+    }
+    if (has_m) {
+      const int pr = trans ? jm : row, pc = trans ? row : jm;
+      if (pr < wk.prows && pc < wk.pcols) wk.partial[(long long)pr * wk.ldp + pc] = im[e];
     }
   }
 }
@@ -319,7 +360,7 @@ template <int PREC, int EPI, int WN, int KS>
 __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void gemm_kernel(
     const Work* __restrict__ works, const Problem* __restrict__ probs, const double* __restrict__ chunk_x) {
   using C = Cfg<WN, KS>;
-  constexpr int BN = C::BN;
+  constexpr int BS = C::BS;
   __shared__ Lds<PREC, WN, KS> lds;
 
   const int tid = threadIdx.x;
@@ -329,34 +370,32 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
   const Work wk = works[blockIdx.x];
   if (wk.pbeg >= wk.pend) return;
 
-  // staging coordinates: A rows from a_row0, one B column / K group
+  // staging coordinates: A rows from a_row0, one slot / K group of twiddles
   const long long a_row0 = (long long)wk.tr * BM;
-  const int st_col = bitem_col(tid);
-  const int st_g = item_part(tid);
-  const int b_col = wk.tc * BN + st_col;
+  const int st_slot = bitem_slot(tid);
+  const int st_g = bitem_group(tid);
+  const int b_slot = wk.tc * BS + st_slot;
 
-  f32x16 cr[2], ci[2], iacc[2];
+  f32x16 acc[4], ip, im;
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    cr[i] = f32x16{};
-    ci[i] = f32x16{};
-    iacc[i] = f32x16{};
-  }
+  for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
+  ip = f32x16{};
+  im = f32x16{};
 
   // ---- prologue: stage chunk 0 of the first problem into buffer 0 ----------
   int sp = wk.pbeg, ss = 0;  // staging cursor: next (problem, chunk) to stage
   StageProb sgp;
   float2 q[C::TW];
   Staged<WN, KS> st;
-  set_stage<KS>(probs[sp], b_col, sgp, q);
+  set_stage<KS>(probs[sp], b_slot, sgp, q);
   double sx = twiddle_x<KS>(chunk_x, sgp, 0, st_g);
   load_a<PREC, WN, KS>(sgp, a_row0, tid, 0, st);
   gen_b<PREC, WN, KS>(sx, sgp, q, st);
-  write_b<PREC, WN, KS>(lds, 0, st_col, st_g, st);
+  write_b<PREC, WN, KS>(lds, 0, st_slot, st_g, st);
   write_a<PREC, WN, KS>(lds, 0, tid, st);
   if (++ss == sgp.nch) {
     ss = 0;
-    if (++sp < wk.pend) set_stage<KS>(probs[sp], b_col, sgp, q);
+    if (++sp < wk.pend) set_stage<KS>(probs[sp], b_slot, sgp, q);
   }
   sx = twiddle_x<KS>(chunk_x, sgp, ss, st_g);
   __syncthreads();
@@ -374,46 +413,30 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
 
       if (PREC == PREC_BF16X3) {
         auto& L = reinterpret_cast<Lds<PREC_BF16X3, WN, KS>&>(lds);
-        const int r0 = wr * 64 + (lane & 31), c0 = wc * 32 + (lane & 31);
+        const int r0 = wr * 32 + (lane & 31), c0 = wc * 32 + (lane & 31);
 #pragma unroll
         for (int j = 0; j < KS; ++j) {
-          // fragment rows are 32-aligned + (lane & 31): one swizzle key per lane
+          // fragment rows are 16-aligned + (lane & 31): one swizzle key per lane
           const int koff = swz<KS>(lane & 31, 2 * j + (lane >> 5));
-          bf16x8_t a0[4], a1[4], b[4], nb[2];
+          bf16x8_t a[4], b[4];
 #pragma unroll
           for (int p = 0; p < 4; ++p) {
-            a0[p] = *(const bf16x8_t*)&L.a[cur][p][r0][koff];
+            a[p] = *(const bf16x8_t*)&L.a[cur][p][r0][koff];
             b[p] = *(const bf16x8_t*)&L.b[cur][p][c0][koff];
           }
-#pragma unroll
-          for (int p = 0; p < 4; ++p) a1[p] = *(const bf16x8_t*)&L.a[cur][p][r0 + 32][koff];
-          nb[0] = neg_bf16x8(b[2]);
-          nb[1] = neg_bf16x8(b[3]);
-          cblock_bf16(a0, b, nb, cr[0], ci[0]);
-          cblock_bf16(a1, b, nb, cr[1], ci[1]);
+          sblock_bf16(a, b, acc);
         }
         // schedule: next chunk's global loads first (consumed only by the LDS
         // stores after the MFMAs), fragment reads, then the twiddle VALU work
         // threaded between the MFMAs
         SGB(SG_VMEM_RD, 2 * C::A_ITEMS);
-        SGB(SG_DSR, 8);
 #pragma unroll
-        for (int i = 0; i < 12; ++i) {
-          SGB(SG_MFMA, 1);
-          SGB(SG_VALU, 4);
-        }
-        SGB(SG_DSR, 4);
+        for (int j = 0; j < KS; ++j) {
+          SGB(SG_DSR, 8);
 #pragma unroll
-        for (int i = 0; i < 12; ++i) {
-          SGB(SG_MFMA, 1);
-          SGB(SG_VALU, 2);
-        }
-        if (KS == 2) {
-          SGB(SG_DSR, 12);
-#pragma unroll
-          for (int i = 0; i < 24; ++i) {
+          for (int i = 0; i < 12; ++i) {
             SGB(SG_MFMA, 1);
-            SGB(SG_VALU, 2);
+            SGB(SG_VALU, 3);
           }
         }
       } else {
@@ -421,43 +444,33 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
 #pragma unroll
         for (int ks = 0; ks < C::BKC / 2; ++ks) {
           const int kk = 2 * ks + (lane >> 5);
-          float2 a[2];
-#pragma unroll
-          for (int i = 0; i < 2; ++i) a[i] = L.a[cur][wr * 64 + i * 32 + (lane & 31)][kk];
+          const float2 a = L.a[cur][wr * 32 + (lane & 31)][kk];
           const float2 b = L.b[cur][wc * 32 + (lane & 31)][kk];
-#pragma unroll
-          for (int i = 0; i < 2; ++i) {
-            f32x16 r = cr[i], m = ci[i];
-            r = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].x, b.x, r, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].x, b.y, m, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].y, -b.y, r, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].y, b.x, m, 0, 0, 0);
-            cr[i] = r;
-            ci[i] = m;
-          }
+          acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc[0], 0, 0, 0);
+          acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.x, acc[1], 0, 0, 0);
+          acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acc[2], 0, 0, 0);
+          acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc[3], 0, 0, 0);
         }
       }
 
-      write_b<PREC, WN, KS>(lds, cur ^ 1, st_col, st_g, st);
+      write_b<PREC, WN, KS>(lds, cur ^ 1, st_slot, st_g, st);
       write_a<PREC, WN, KS>(lds, cur ^ 1, tid, st);
       __syncthreads();
       cur ^= 1;
       if (more) {
         if (++ss == sgp.nch) {
           ss = 0;
-          if (++sp < wk.pend) set_stage<KS>(probs[sp], b_col, sgp, q);
+          if (++sp < wk.pend) set_stage<KS>(probs[sp], b_slot, sgp, q);
         }
         sx = twiddle_x<KS>(chunk_x, sgp, sp < wk.pend ? ss : 0, st_g);
       }
     }
-    problem_epilogue<PREC, EPI>(probs[pc], wk, BN, wr, wc, lane, cr, ci, iacc);
+    problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      cr[i] = f32x16{};
-      ci[i] = f32x16{};
-    }
+    for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
   }
-  if (EPI == EPI_INTENSITY) store_partial(wk, probs[wk.pbeg].trans, BN, wr, wc, lane, iacc);
+  if (EPI == EPI_INTENSITY)
+    store_partial(wk, probs[wk.pbeg], BS, wr, wc, lane, ip, im);
 }
 
 template <int WN, int KS>
@@ -482,8 +495,8 @@ static hipError_t launch_cfg(int prec, int epi, const Work* work, int nwork, con
 hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, const Problem* probs,
                        const double* chunk_x, hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
-  if (bn == 64) return launch_cfg<2, 1>(prec, epi, work, nwork, probs, chunk_x, s);
-  if (bn == 128) return launch_cfg<4, 2>(prec, epi, work, nwork, probs, chunk_x, s);
+  if (bn == 64) return launch_cfg<1, 1>(prec, epi, work, nwork, probs, chunk_x, s);
+  if (bn == 128) return launch_cfg<2, 2>(prec, epi, work, nwork, probs, chunk_x, s);
   return hipErrorInvalidValue;
 }
 

@@ -5,17 +5,19 @@
 
 namespace mftx {
 
-// Tile geometry of the complex GEMM C = A . B (A data, B generated twiddles):
-// 2 x WN waves per workgroup, each wave a 64 x 32 block of C; C tile
-// BM x BN with BN = 32 * WN, WN = 2 (BN 64) or 4 (BN 128).
-constexpr int WAVES_M = 2;
-constexpr int BM = 64 * WAVES_M;   // C rows per workgroup (A rows)
+// Tile geometry of the complex GEMM: WAVES_M x WN waves per workgroup, each
+// wave a 32-row x 32-slot block (64 output columns, see Problem); BM rows and
+// 32 WN slots per workgroup.
+constexpr int WAVES_M = 4;
+constexpr int BM = 32 * WAVES_M;   // C rows per workgroup (A rows)
 constexpr int BK = 16;             // K block: granule of the packed layout and of chunk_x
-constexpr int BN_DEFAULT = 128;    // C cols per workgroup unless a plan picks another
+constexpr int BN_DEFAULT = 128;    // output columns per workgroup unless a plan picks another
+// output columns per workgroup: 64 (32 slots, 16-wide K steps) or 128 (64 slots, 32-wide K steps)
 inline bool valid_bn(int bn) { return bn == 64 || bn == 128; }
-// K per GEMM pipeline stage for a tile width (64 columns: 16, 128 columns: 32);
-// every GEMM problem's kpad is a multiple of it.
+// K per GEMM pipeline stage for a tile width; every GEMM problem's kpad is a multiple of it.
 inline int k_granule(int bn) { return bn == 128 ? 32 : 16; }
+// slots per workgroup for a tile width
+inline int slots_per_tile(int bn) { return bn / 2; }
 static_assert(BM == 128, "staging assumes 128 A rows per workgroup");
 
 enum Precision { PREC_FP32 = 0, PREC_BF16X3 = 1 };
@@ -27,32 +29,50 @@ enum Epilogue {
   EPI_INTENSITY = 2,    // accumulate w*|C|^2 over the workgroup's problems into a partial image
 };
 
-// One complex GEMM problem.  Twiddle B[k][j] = cis(alpha * x_k * y_j) with
-// y_j = y0 + j and x_k = chunk_x[chunk_begin + k/BK] + (k % BK): x advances
-// by one per K index inside each BK-wide chunk, and each chunk carries its
-// own starting x (so several K segments with their own coordinate origins
-// can be concatenated); cis(t) = exp(2*pi*i*t).
+// One complex GEMM problem: one MFT stage,
+//   C[i][j] = sum_k A[i][k] cis(alpha x_k (j - N/2 - shift)),   0 <= j < N = ncols,
+// with cis(t) = exp(2 pi i t) and N/2 the integer half, computed about a
+// pairing centre c (integer or half-integer; the host picks it at the PSF core
+// j = N/2 + shift or at the grid centre next to it).  The producer of A has
+// multiplied A[i][k] by the fold cis(-alpha (shift + N/2 - c) x_k), so that
+// with Y = j - c
+//   C[i][j] = sum_k A'[i][k] cis(alpha x_k Y),
+// whose cosine part is even and sine part odd in Y.  The GEMM runs over slots
+// s = 0 .. cols-1 with Y_s = y0 + s (y0 = 0 for integer c, 1/2 otherwise):
+// P = sum_k A' cos, Q = sum_k A' sin, and
+//   C[i][cp + s] = P + i Q,    C[i][cm - s] = P - i Q
+// (cp = ceil(c), cm = floor(c); outputs outside [0, N) are dropped, and the
+// second is dropped where it equals the first).  c = 0 is the direct product
+// (one output per slot).  Twiddle B[k][s] = cis(alpha * x_k * Y_s) with
+// x_k = chunk_x[chunk_begin + k/BK] + (k % BK): x advances by one per K index
+// inside each BK-wide block, and each block carries its own starting x (so
+// several K segments with their own coordinate origins can be concatenated).
 struct Problem {
-  const void* a;        // packed A operand (row-major, K-blocked)
+  const void* a;        // packed A' operand (row-major, K-blocked)
   long long lda;        // A row stride in scalar elements (float2 for fp32, bf16 for bf16x3)
   double alpha;         // signed twiddle scale in turns per unit of x*y
-  double y0;            // y of column 0
-  int rows, cols;       // valid C extent
+  double y0;            // Y of slot 0
+  int rows;             // valid C rows
+  int cols;             // slots
+  int ncols;            // N: output columns
+  int cp, cm;           // output columns of slot 0 (P + iQ, P - iQ)
   int kpad;             // K (multiple of the launch's k_granule); A is zero beyond the valid K
   int chunk_begin;      // index of this problem's first entry in chunk_x
   float scale;          // store: C *= scale; intensity: weight applied to |C|^2
   int limit;            // EPI_STORE_PACKED_T: C rows written (K extent of the destination)
+  int trans;            // EPI_STORE_CPLX / EPI_INTENSITY: write C transposed
   void* dst;            // store destination (EPI_STORE_*)
   long long ldd;        // destination row stride (elements of its type)
-  int trans;            // EPI_STORE_CPLX / EPI_INTENSITY: write C transposed
-  int pad_;
+  double fold;          // EPI_STORE_PACKED_T: C row r is stored times cis(fold * (fold_x0 + r)),
+  double fold_x0;       //   the next stage's fold
 };
 
-// A contiguous K range with its own coordinate origin; the host expands
-// segments into the per-chunk chunk_x table.
-struct Segment {
-  int kbeg, kend;       // [kbeg, kend), multiples of BK
-  double x0;            // x at k = kbeg
+// Per-wavelength parameters of a phasor launch: the phase opd * inv_wl and the
+// stage-1 fold cis(fold * (x0 + k)) along the operand's K index k.
+struct PhasorWave {
+  double inv_wl;
+  double fold;
+  double x0;
 };
 
 // A workgroup's unit of work: problems [pbeg, pend) on tile (tr, tc) of C.
@@ -76,15 +96,17 @@ __host__ __device__ inline size_t packed_bytes(int prec, long long rows, int kpa
   return (size_t)rows * (size_t)kpad * (prec == PREC_FP32 ? 8 : 8);
 }
 
-// launchers (mftx_kernels.hip)
-// bn: C columns per workgroup (64 or 128, see valid_bn).
+// launchers (mftx_gemm.hip, mftx_kernels.hip)
+// bn: output columns per workgroup (64 or 128, see valid_bn).
 hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, const Problem* probs,
                        const double* chunk_x, hipStream_t s);
+// packed A' = src (or src^T) times the fold cis(fold * (x0 + k)) along K
 hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n,
-                       long long lds, int trans, void* dst, long long rows_pad, int kpad, hipStream_t s);
+                       long long lds, int trans, double fold, double x0, void* dst, long long rows_pad, int kpad,
+                       hipStream_t s);
 hipError_t launch_phasor(int prec, const void* amp, int amp_dtype, long long ld_amp,
                          const void* opd, int opd_dtype, long long ld_opd, int m, int n, int trans,
-                         const double* inv_wavelengths_dev, int nwave, void* dst,
+                         const PhasorWave* waves_dev, int nwave, void* dst,
                          long long dst_wave_stride, long long rows_pad, int kpad, hipStream_t s);
 hipError_t launch_phasor_field(const void* amp, int amp_dtype, long long ld_amp, const void* opd,
                                int opd_dtype, long long ld_opd, int m, int n, double inv_wavelength,

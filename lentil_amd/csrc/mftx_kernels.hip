@@ -48,10 +48,11 @@ __device__ __forceinline__ void store_packed16(int prec, void* dst, long long ld
 
 // One thread per (row, 16-wide K block) of the packed destination.  The
 // source is m x n with row stride lds; packed element (row, k) is source
-// (row, k), or source (k, row) when trans (the packed operand is then n x m).
-// Everything outside the source is zero.
+// (row, k), or source (k, row) when trans (the packed operand is then n x m),
+// times the fold cis(fold * (x0 + k)).  Everything outside the source is zero.
 __global__ void pack_kernel(int prec, const void* __restrict__ src, int dtype, long long m, long long n,
-                            long long lds, int trans, void* __restrict__ dst, long long rows_pad, int kpad) {
+                            long long lds, int trans, double fold, double x0, void* __restrict__ dst,
+                            long long rows_pad, int kpad) {
   const int kblocks = kpad / 16;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= rows_pad * kblocks) return;
@@ -62,18 +63,23 @@ __global__ void pack_kernel(int prec, const void* __restrict__ src, int dtype, l
   for (int i = 0; i < 16; ++i) {
     const long long k = (long long)kb * 16 + i;
     const long long r = trans ? k : row, c = trans ? row : k;
-    v[i] = (r < m && c < n) ? load_elem(src, dtype, r * lds + c) : make_float2(0.f, 0.f);
+    if (r < m && c < n) {
+      const float2 e = load_elem(src, dtype, r * lds + c);
+      v[i] = fold == 0.0 ? e : cmul(e, cis_turns(fold * (x0 + (double)k)));
+    } else {
+      v[i] = make_float2(0.f, 0.f);
+    }
   }
   store_packed16(prec, dst, packed_lda(prec, kpad), row, kb, v);
 }
 
 hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n, long long lds,
-                       int trans, void* dst, long long rows_pad, int kpad, hipStream_t s) {
+                       int trans, double fold, double x0, void* dst, long long rows_pad, int kpad, hipStream_t s) {
   const long long total = rows_pad * (kpad / 16);
   if (total <= 0) return hipSuccess;
   const int bs = 256;
   hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, src,
-                     src_dtype, m, n, lds, trans, dst, rows_pad, kpad);
+                     src_dtype, m, n, lds, trans, fold, x0, dst, rows_pad, kpad);
   return hipGetLastError();
 }
 
@@ -87,7 +93,9 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
 
 // One thread per (row, 16-wide K block, group of PH_WAVES wavelengths):
 // the 16 amp/opd values are read once and the packed operand of the m x n
-// tile (n x m when trans) is written for each wavelength of the group.
+// tile (n x m when trans) is written for each wavelength of the group, with
+// that wavelength's stage-1 fold along K folded into the same phase:
+//   amp * cis(opd * inv_wl + fold * (x0 + k)).
 // Consecutive threads take consecutive source columns (K blocks when not
 // trans, rows of the transposed operand when trans), so the source reads of
 // a wave are contiguous.
@@ -95,7 +103,7 @@ constexpr int PH_WAVES = 8;
 
 __global__ void phasor_kernel(int prec, const void* __restrict__ amp, int amp_dtype, long long ld_amp,
                               const void* __restrict__ opd, int opd_dtype, long long ld_opd, int m, int n,
-                              int trans, const double* __restrict__ inv_wl, int nwave, void* __restrict__ dst,
+                              int trans, const PhasorWave* __restrict__ waves, int nwave, void* __restrict__ dst,
                               long long wave_stride, long long rows_pad, int kpad) {
   const int kblocks = kpad / 16;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,11 +127,12 @@ __global__ void phasor_kernel(int prec, const void* __restrict__ amp, int amp_dt
   const long long lda = packed_lda(prec, kpad);
   const int w_end = min(nwave, (g + 1) * PH_WAVES);
   for (int w = g * PH_WAVES; w < w_end; ++w) {
-    const double iw = inv_wl[w];
+    const PhasorWave pw = waves[w];
+    const double f0 = pw.fold * (pw.x0 + (double)kb * 16);
     float2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-      const float2 c = cis_turns(o[i] * iw);
+      const float2 c = cis_turns(__builtin_fma(o[i], pw.inv_wl, __builtin_fma(pw.fold, (double)i, f0)));
       v[i] = make_float2(a[i] * c.x, a[i] * c.y);
     }
     store_packed16(prec, (char*)dst + (long long)w * wave_stride, lda, row, kb, v);
@@ -131,13 +140,13 @@ __global__ void phasor_kernel(int prec, const void* __restrict__ amp, int amp_dt
 }
 
 hipError_t launch_phasor(int prec, const void* amp, int amp_dtype, long long ld_amp, const void* opd,
-                         int opd_dtype, long long ld_opd, int m, int n, int trans, const double* inv_wl, int nwave,
+                         int opd_dtype, long long ld_opd, int m, int n, int trans, const PhasorWave* waves, int nwave,
                          void* dst, long long wave_stride, long long rows_pad, int kpad, hipStream_t s) {
   const long long total = rows_pad * (kpad / 16) * ((nwave + PH_WAVES - 1) / PH_WAVES);
   if (total <= 0) return hipSuccess;
   const int bs = 256;
   hipLaunchKernelGGL(phasor_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, amp,
-                     amp_dtype, ld_amp, opd, opd_dtype, ld_opd, m, n, trans, inv_wl, nwave, dst, wave_stride,
+                     amp_dtype, ld_amp, opd, opd_dtype, ld_opd, m, n, trans, waves, nwave, dst, wave_stride,
                      rows_pad, kpad);
   return hipGetLastError();
 }
