@@ -28,7 +28,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {'bf16x3': 2.5e15, 'fp32': 157.3e12}   # dense MFMA peaks of MI355X (FLOP/s)
-MFMA_PASSES = {'bf16x3': 3, 'fp32': 1}        # executed MFMA work / algorithmic work
 HBM_PEAK = 8e12                               # MI355X HBM3E bytes/s
 
 
@@ -38,7 +37,10 @@ def parse():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--precision', default='bf16x3', choices=['bf16x3', 'fp32'])
-    ap.add_argument('--lambda-group', type=int, default=4)
+    ap.add_argument('--lambda-group', type=int, default=0,
+                    help='wavelengths per register accumulation (0: chosen by the autotuner)')
+    ap.add_argument('--tile-n', type=int, default=0, choices=[0, 64, 128],
+                    help='output columns per GEMM workgroup (0: chosen by the autotuner)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the FP64 error check')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain per-wavelength FP32 path')
     ap.add_argument('--copy-mib', type=int, default=0,
@@ -50,7 +52,7 @@ def parse():
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'broadband_512_stage2_hbm_traffic.md')
 
 
-def measured_traffic(precision, lambda_group):
+def measured_traffic(precision, lambda_group, tile_n):
     """HBM bytes per stage-2 launch measured with rocprofv3 --pmc (FETCH_SIZE
     and WRITE_SIZE, scaled by a plain copy of known size), read from the
     committed counter summary; None when it holds no figure for this
@@ -60,8 +62,8 @@ def measured_traffic(precision, lambda_group):
         text = open(TRAFFIC_FILE).read()
     except OSError:
         return None
-    m = re.search(rf'^traffic_bytes_per_launch precision={precision} lambda_group={lambda_group}: (\d+)', text,
-                  re.M)
+    m = re.search(rf'^traffic_bytes_per_launch precision={precision} lambda_group={lambda_group} '
+                  rf'tile_n={tile_n}: (\d+)', text, re.M)
     return int(m.group(1)) if m else None
 
 
@@ -111,9 +113,10 @@ def main():
     cfg = workloads.broadband_512()
     nwave = len(cfg['wavelengths'])
     pupil = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'], focal_length=cfg['focal_length'])
+    tuned = args.lambda_group == 0 and args.tile_n == 0
     plan = L.PSFPlan(pupil, cfg['wavelengths'], cfg['weights'], pixelscale=cfg['pixelscale'], shape=cfg['shape'],
-                     oversample=cfg['oversample'], precision=args.precision, lambda_group=args.lambda_group,
-                     use_graph=True)
+                     oversample=cfg['oversample'], precision=args.precision, lambda_group=args.lambda_group or 8,
+                     tile_n=args.tile_n, use_graph=True, autotune=tuned)
     out = torch.empty(plan.shape_out, dtype=torch.float32, device='cuda')
     for _ in range(args.warmup):
         plan(out)
@@ -128,23 +131,33 @@ def main():
     image = out.double().cpu().numpy()
     print(f'[bench] {args.precision}: {ms_step:.3f} ms/image, {psfs_per_s:.1f} PSFs/s', file=sys.stderr, flush=True)
     if args.copy_mib > 0:
+        # a plain copy of known size whose source was pushed out of the caches
+        # (1 GiB written in between) scales the hardware byte counters
         src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device='cuda')
+        evict = torch.ones(1 << 28, dtype=torch.float32, device='cuda')
         dst = torch.empty_like(src)
+        torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        del src, dst
+        del src, dst, evict
 
     # dominant kernel: stage-2 GEMM, HIP events on the plan's stream
     stage_ms = plan.profile(torch.empty_like(out), repeats=5)
     flops2 = plan.info['gemm_flops'][1]
     ach = flops2 / (stage_ms['stage2'] * 1e-3)
     total_flops = plan.info['flops']
+    mfma2 = plan.info['mfma_flops'][1]
     roofline = {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK[args.precision] / 1e12, 'unit': 'TFLOP/s',
-                'frac': ach / PEAK[args.precision], 'traffic': measured_traffic(args.precision, args.lambda_group),
+                'frac': ach / PEAK[args.precision],
+                'traffic': measured_traffic(args.precision, plan.lambda_group, plan.tile_n),
                 'traffic_source': os.path.relpath(TRAFFIC_FILE, ROOT),
-                'kernel': 'gemm_kernel<bf16x3|fp32, intensity> (stage 2)',
+                'kernel': f'gemm_kernel<{args.precision}, intensity> (stage 2)',
                 'algorithmic_flops_per_launch': flops2, 'avg_launch_ms': stage_ms['stage2'],
-                'mfma_pipe_frac': ach * MFMA_PASSES[args.precision] / PEAK[args.precision]}
+                'executed_mfma_flops_per_launch': mfma2,
+                'mfma_pipe_frac': mfma2 / (stage_ms['stage2'] * 1e-3) / PEAK[args.precision],
+                'stage1': {'algorithmic_tflops': plan.info['gemm_flops'][0] / (stage_ms['stage1'] * 1e-3) / 1e12,
+                           'mfma_pipe_frac': plan.info['mfma_flops'][0] / (stage_ms['stage1'] * 1e-3)
+                           / PEAK[args.precision]}}
     print(f'[bench] stages ms {stage_ms}', file=sys.stderr, flush=True)
     # HBM-bound field kernels: algorithmic bytes / event-timed duration
     field_kernels = {k: {'ms': stage_ms[k], 'bytes': plan.info[f'bytes_{k}'],
@@ -153,6 +166,9 @@ def main():
                      for k in ('phasor', 'reduce')}
 
     extra = {}
+    if tuned:
+        extra['autotune_ms_per_image'] = {f'lambda_group={lg},tile_n={tn}': round(t, 4)
+                                          for (lg, tn), t in sorted(plan.tuning.items())}
     if not args.no_plain:
         # plain per-wavelength FP32 path: the reference user loop through the drop-in API, device-resident image
         img_dev = torch.zeros(plan.shape_out, dtype=torch.float64, device='cuda')
@@ -172,7 +188,8 @@ def main():
         plain_img = img_dev.cpu().numpy()
     if args.precision != 'fp32':
         p32 = L.PSFPlan(pupil, cfg['wavelengths'], cfg['weights'], pixelscale=cfg['pixelscale'], shape=cfg['shape'],
-                        oversample=cfg['oversample'], precision='fp32', lambda_group=args.lambda_group)
+                        oversample=cfg['oversample'], precision='fp32', lambda_group=plan.lambda_group,
+                        tile_n=plan.tile_n)
         o32 = torch.empty_like(out)
         p32(o32)
         torch.cuda.synchronize()
@@ -204,8 +221,8 @@ def main():
         'ms_per_step': ms_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': args.precision, 'data': 'synthetic (seeded pupil, Zernike OPD)',
         'config': {'workload': 'broadband_512', 'pupil': 512, 'detector': 1024, 'wavelengths': nwave,
-                   'psfs_per_step': nwave, 'precision': args.precision, 'lambda_group': args.lambda_group,
-                   'parallelism': 'single GPU'},
+                   'psfs_per_step': nwave, 'precision': args.precision, 'lambda_group': plan.lambda_group,
+                   'tile_n': plan.tile_n, 'autotuned': tuned, 'parallelism': 'single GPU'},
         'tflops_algorithmic': total_flops * args.steps / dt / 1e12,
         'stage_ms': stage_ms,
         'roofline': roofline,

@@ -168,6 +168,7 @@ typedef struct {
 typedef struct {
   double flops;          /* algorithmic FLOPs of one execute (8 per complex MAC) */
   double gemm_flops[2];  /* per GEMM stage */
+  double mfma_flops[2];  /* executed MFMA FLOPs per GEMM stage (split passes and tile padding included) */
   double bytes_hbm_min;  /* minimum HBM bytes of one execute (phasor + intermediates + image) */
   double bytes_phasor;   /* algorithmic bytes of the phasor launches (amp/opd read once, operands written) */
   double bytes_reduce;   /* algorithmic bytes of the detector reductions (partials read, image written) */

@@ -41,7 +41,7 @@ class MftxProblem(ctypes.Structure):
 
 
 class MftxPlanInfo(ctypes.Structure):
-    _fields_ = [('flops', c_dbl), ('gemm_flops', c_dbl * 2), ('bytes_hbm_min', c_dbl),
+    _fields_ = [('flops', c_dbl), ('gemm_flops', c_dbl * 2), ('mfma_flops', c_dbl * 2), ('bytes_hbm_min', c_dbl),
                 ('bytes_phasor', c_dbl), ('bytes_reduce', c_dbl),
                 ('n_launch', ctypes.c_int32), ('coherent_fast', ctypes.c_int32),
                 ('n_stage1', ctypes.c_int32), ('n_stage2', ctypes.c_int32),

@@ -188,7 +188,7 @@ class PSFPlan:
         _lib.load().mftx_plan_set_graph(self._plan, 1 if use_graph else 0)
         info = _lib.MftxPlanInfo()
         _lib.load().mftx_plan_get_info(self._plan, ctypes.byref(info))
-        self.info = {k: (list(getattr(info, k)) if k == 'gemm_flops' else getattr(info, k))
+        self.info = {k: (list(getattr(info, k)) if k in ('gemm_flops', 'mfma_flops') else getattr(info, k))
                      for k, _ in _lib.MftxPlanInfo._fields_}
 
     def _build(self, lambda_group, tile_n):

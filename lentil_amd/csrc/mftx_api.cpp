@@ -746,6 +746,15 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     }
   }
   std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
+  // executed MFMA FLOPs of the two launches: per workgroup and problem, BM rows
+  // x slots x kpad x 4 real products (re/im x cos/sin) x 2, times the passes
+  const double passes = prec == PREC_BF16X3 ? 3.0 : 1.0;
+  const double per_slot_k = 8.0 * BM * slots_per_tile(tile_n) * passes;
+  double mf1 = 0, mf2 = 0;
+  for (auto& w : w1)
+    for (int i = w.pbeg; i < w.pend; ++i) mf1 += per_slot_k * p1[i].kpad;
+  for (auto& w : w2)
+    for (int i = w.pbeg; i < w.pend; ++i) mf2 += per_slot_k * p2_launch[i].kpad;
 
   const size_t o_w1 = lay.take(sizeof(Work) * std::max<size_t>(w1.size(), 1));
   const size_t o_w2 = lay.take(sizeof(Work) * std::max<size_t>(w2.size(), 1));
@@ -849,6 +858,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->info.flops = fl;
   plan->info.gemm_flops[0] = g1;
   plan->info.gemm_flops[1] = g2;
+  plan->info.mfma_flops[0] = mf1;
+  plan->info.mfma_flops[1] = mf2;
   double bytes = 0;
   for (auto& ph : plan->phasors) bytes += 2.0 * (double)ph.wave_stride * ph.nwave;  // write + read
   for (auto& c : cls) bytes += 2.0 * packed_bytes(prec, c.rows2, c.ktot);

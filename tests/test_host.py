@@ -42,7 +42,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.MftxTile) == 8 * 2 + 4 * 2 + 8 * 4
     # mftx_problem: 5 int32 (+4 pad), 6 doubles, 2 int64, 1 double
     assert ctypes.sizeof(_lib.MftxProblem) == 5 * 4 + 4 + 8 * 6 + 8 * 2 + 8
-    assert ctypes.sizeof(_lib.MftxPlanInfo) == 8 * 6 + 4 * 6
+    assert ctypes.sizeof(_lib.MftxPlanInfo) == 8 * 8 + 4 * 6
 
 
 def test_error_mapping():
