@@ -139,15 +139,9 @@ struct mftx_plan {
   PhasorWave* waves = nullptr;  // per phasor wavelength slot: 1/λ and the stage-1 fold
   int nwork1 = 0, nwork2 = 0;
   int epi2 = EPI_INTENSITY;
-  struct Phasor {
-    mftx_tile tile;
-    int trans, nwave, wave_off;
-    void* dst;
-    long long wave_stride;  // bytes
-    long long rows_pad;
-    int kpad;
-  };
-  std::vector<Phasor> phasors;
+  PhasorOp* phasor_ops = nullptr;  // device descriptors of the grouped phasor launch
+  int n_phasor_ops = 0;
+  long long phasor_threads = 0;
   struct Reduce {
     float* partial;
     long long gstride, ldp;
@@ -762,6 +756,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   const size_t o_p2 = lay.take(sizeof(Problem) * std::max<size_t>(p2_launch.size(), 1));
   const size_t o_cx = lay.take(sizeof(double) * std::max<size_t>(cx.size(), 1));
   const size_t o_iw = lay.take(sizeof(PhasorWave) * std::max<size_t>(wave_slots.size(), 1));
+  const size_t o_ops = lay.take(sizeof(PhasorOp) * std::max<size_t>(tops.size(), 1));
   std::vector<size_t> fb_off;
   if (!fast)
     for (size_t g = 0; g < groups.size(); ++g) fb_off.push_back(lay.take(sizeof(float2) * image_rows * image_cols));
@@ -811,18 +806,39 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     return hipfail(ctx, e, "plan descriptor upload");
   }
 
+  std::vector<PhasorOp> ops;
+  long long pthreads = 0;
   for (auto& kv : tops) {
     const TileOp& op = kv.second;
-    mftx_plan::Phasor ph;
-    ph.tile = tiles[kv.first.first];
+    const mftx_tile& T = tiles[kv.first.first];
+    PhasorOp ph = {};
+    ph.amp = T.amp;
+    ph.opd = T.opd;
+    ph.amp_dtype = T.amp_dtype;
+    ph.opd_dtype = T.opd_dtype;
+    ph.ld_amp = T.ld_amp;
+    ph.ld_opd = T.ld_opd;
+    ph.m = (int)T.m;
+    ph.n = (int)T.n;
     ph.trans = kv.first.second;
     ph.nwave = (int)op.waves.size();
-    ph.wave_off = op.wave_off;
+    ph.waves = plan->waves + op.wave_off;
     ph.dst = base + op.off;
     ph.wave_stride = (long long)packed_bytes(prec, op.rows, op.kpad);
     ph.rows_pad = op.rows;
     ph.kpad = op.kpad;
-    plan->phasors.push_back(ph);
+    ph.begin = pthreads;
+    pthreads += phasor_threads(ph.rows_pad, ph.kpad, ph.nwave);
+    ops.push_back(ph);
+  }
+  plan->phasor_ops = (PhasorOp*)(base + o_ops);
+  plan->n_phasor_ops = (int)ops.size();
+  plan->phasor_threads = pthreads;
+  e = hipMemcpy(plan->phasor_ops, ops.data(), sizeof(PhasorOp) * ops.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(plan->ws);
+    delete plan;
+    return hipfail(ctx, e, "plan descriptor upload");
   }
   if (fast) {
     for (auto& gi : geos) {
@@ -861,13 +877,13 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->info.mfma_flops[0] = mf1;
   plan->info.mfma_flops[1] = mf2;
   double bytes = 0;
-  for (auto& ph : plan->phasors) bytes += 2.0 * (double)ph.wave_stride * ph.nwave;  // write + read
+  for (auto& ph : ops) bytes += 2.0 * (double)ph.wave_stride * ph.nwave;  // write + read
   for (auto& c : cls) bytes += 2.0 * packed_bytes(prec, c.rows2, c.ktot);
   bytes += 4.0 * image_rows * image_cols;
   plan->info.bytes_hbm_min = bytes;
   double bph = 0, bre = 0;
-  for (auto& ph : plan->phasors)
-    bph += (double)ph.tile.m * ph.tile.n * ((ph.tile.amp_dtype == MFTX_DTYPE_F64 ? 8.0 : 4.0) + (ph.tile.opd_dtype == MFTX_DTYPE_F64 ? 8.0 : 4.0)) +
+  for (auto& ph : ops)
+    bph += (double)ph.m * ph.n * ((ph.amp_dtype == MFTX_DTYPE_F64 ? 8.0 : 4.0) + (ph.opd_dtype == MFTX_DTYPE_F64 ? 8.0 : 4.0)) +
            (double)ph.wave_stride * ph.nwave;
   for (auto& r : plan->reduces) {
     const double vr = (double)std::max<int64_t>(0, std::min<int64_t>(r.r0 + r.rows, image_rows) - std::max<int64_t>(r.r0, 0));
@@ -881,7 +897,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->info.n_stage2 = (int)cls.size();
   plan->info.n_work1 = plan->nwork1;
   plan->info.n_work2 = plan->nwork2;
-  plan->info.n_launch = (int)plan->phasors.size() + 2 + (int)plan->reduces.size();
+  plan->info.n_launch = (ops.empty() ? 0 : 1) + 2 + (int)plan->reduces.size();
   *out = plan;
   return MFTX_OK;
 }
@@ -889,13 +905,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
 }  // extern "C"
 
 static hipError_t launch_phasors(const mftx_plan* plan, hipStream_t s) {
-  for (auto& ph : plan->phasors) {
-    hipError_t e = launch_phasor(plan->prec, ph.tile.amp, ph.tile.amp_dtype, ph.tile.ld_amp, ph.tile.opd,
-                                 ph.tile.opd_dtype, ph.tile.ld_opd, (int)ph.tile.m, (int)ph.tile.n, ph.trans,
-                                 plan->waves + ph.wave_off, ph.nwave, ph.dst, ph.wave_stride, ph.rows_pad, ph.kpad, s);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return launch_phasor_group(plan->prec, plan->phasor_ops, plan->n_phasor_ops, plan->phasor_threads, s);
 }
 
 static hipError_t launch_reduces(const mftx_plan* plan, void* image, int dtype, int64_t ldi, int accumulate,

@@ -125,11 +125,23 @@ __device__ __forceinline__ void set_stage(const Problem& P, int slot, StageProb&
 }
 
 // x of the first twiddle of group g of chunk c (chunk_x holds one origin per
-// 16-wide K block).
+// 16-wide K block).  The block origins are read at a wave-uniform address
+// (scalar loads through the scalar cache) and selected per lane: a per-lane
+// index would make this a vector load whose latency sits on every chunk's
+// critical path.
+__device__ __forceinline__ double uniform_double(const double* p) {
+  const long long b = *(const long long*)p;
+  const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int KS>
 __device__ __forceinline__ double twiddle_x(const double* __restrict__ chunk_x, const StageProb& sp, int c, int g) {
   const int k = 2 * KS * g;
-  return chunk_x[sp.cb + KS * c + (k >> 4)] + (double)(k & 15);
+  const double* px = chunk_x + __builtin_amdgcn_readfirstlane(sp.cb + KS * c);  // uniform by construction
+  const double x0 = uniform_double(px);
+  const double xb = KS == 2 ? ((k >> 4) ? uniform_double(px + 1) : x0) : x0;
+  return xb + (double)(k & 15);
 }
 
 template <int PREC, int WN, int KS>
@@ -221,7 +233,7 @@ __device__ __forceinline__ void write_a(Lds<PREC, WN, KS>& L, int buf, int tid, 
 
 #define MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16((a), (b), (c), 0, 0, 0)
 #define SGB(mask, n) __builtin_amdgcn_sched_group_barrier((mask), (n), 0)
-constexpr int SG_VALU = 0x002, SG_MFMA = 0x008, SG_VMEM_RD = 0x020, SG_DSR = 0x100;
+constexpr int SG_VALU = 0x002, SG_MFMA = 0x008, SG_VMEM_RD = 0x020, SG_DSR = 0x100, SG_DSW = 0x200;
 
 // One 32x32 slot block, K = 16: P += A cos, Q += A sin for A = (re, im), each
 // real product as hi*hi + hi*lo + lo*hi.  a: planes (re_hi, re_lo, im_hi,
@@ -382,30 +394,43 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
   ip = f32x16{};
   im = f32x16{};
 
-  // ---- prologue: stage chunk 0 of the first problem into buffer 0 ----------
+  // ---- prologue: chunk 0 into LDS buffer 0, chunk 1 staged in registers ----
   int sp = wk.pbeg, ss = 0;  // staging cursor: next (problem, chunk) to stage
   StageProb sgp;
   float2 q[C::TW];
   Staged<WN, KS> st;
   set_stage<KS>(probs[sp], b_slot, sgp, q);
   double sx = twiddle_x<KS>(chunk_x, sgp, 0, st_g);
+  auto advance = [&]() {
+    if (++ss == sgp.nch) {
+      ss = 0;
+      if (++sp < wk.pend) set_stage<KS>(probs[sp], b_slot, sgp, q);
+    }
+    sx = twiddle_x<KS>(chunk_x, sgp, sp < wk.pend ? ss : 0, st_g);
+  };
   load_a<PREC, WN, KS>(sgp, a_row0, tid, 0, st);
   gen_b<PREC, WN, KS>(sx, sgp, q, st);
   write_b<PREC, WN, KS>(lds, 0, st_slot, st_g, st);
   write_a<PREC, WN, KS>(lds, 0, tid, st);
-  if (++ss == sgp.nch) {
-    ss = 0;
-    if (++sp < wk.pend) set_stage<KS>(probs[sp], b_slot, sgp, q);
+  advance();
+  {
+    // chunk 1 (or, past the last chunk, a valid chunk restaged, never read)
+    const int ts = sp < wk.pend ? ss : 0;
+    load_a<PREC, WN, KS>(sgp, a_row0, tid, ts, st);
+    gen_b<PREC, WN, KS>(sx, sgp, q, st);
+    if (sp < wk.pend) advance();
   }
-  sx = twiddle_x<KS>(chunk_x, sgp, ss, st_g);
   __syncthreads();
 
   int cur = 0;
   for (int pc = wk.pbeg; pc < wk.pend; ++pc) {
     const int nch = probs[pc].kpad / C::BKC;
     for (int s = 0; s < nch; ++s) {
-      // stage the next chunk (after the last one: restage a valid chunk into
-      // the idle buffer, never read)
+      // The registers hold chunk s+1: store it into the idle buffer first (its
+      // LDS stores then overlap this chunk's MFMAs), then stage chunk s+2 into
+      // the registers (past the last chunk: a valid chunk restaged, never read).
+      write_b<PREC, WN, KS>(lds, cur ^ 1, st_slot, st_g, st);
+      write_a<PREC, WN, KS>(lds, cur ^ 1, tid, st);
       const bool more = sp < wk.pend;
       const int ts = more ? ss : 0;
       load_a<PREC, WN, KS>(sgp, a_row0, tid, ts, st);
@@ -426,9 +451,10 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
           }
           sblock_bf16(a, b, acc);
         }
-        // schedule: next chunk's global loads first (consumed only by the LDS
-        // stores after the MFMAs), fragment reads, then the twiddle VALU work
-        // threaded between the MFMAs
+        // schedule: the LDS stores of chunk s+1, the global loads of chunk s+2
+        // (consumed a whole chunk later), fragment reads, then the twiddle VALU
+        // work threaded between the MFMAs
+        SGB(SG_DSW, (KS == 1 ? 4 : 4) + 2 * C::A_ITEMS);
         SGB(SG_VMEM_RD, 2 * C::A_ITEMS);
 #pragma unroll
         for (int j = 0; j < KS; ++j) {
@@ -453,17 +479,9 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
         }
       }
 
-      write_b<PREC, WN, KS>(lds, cur ^ 1, st_slot, st_g, st);
-      write_a<PREC, WN, KS>(lds, cur ^ 1, tid, st);
       __syncthreads();
       cur ^= 1;
-      if (more) {
-        if (++ss == sgp.nch) {
-          ss = 0;
-          if (++sp < wk.pend) set_stage<KS>(probs[sp], b_slot, sgp, q);
-        }
-        sx = twiddle_x<KS>(chunk_x, sgp, sp < wk.pend ? ss : 0, st_g);
-      }
+      if (more) advance();
     }
     problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im);
 #pragma unroll

@@ -96,6 +96,26 @@ __host__ __device__ inline size_t packed_bytes(int prec, long long rows, int kpa
   return (size_t)rows * (size_t)kpad * (prec == PREC_FP32 ? 8 : 8);
 }
 
+// One phasor operand of a plan: the packed (transposed when trans) operand
+// of an m x n tile for nwave wavelengths, written wave_stride bytes apart.
+// `begin` is the op's first thread in the grouped launch.
+constexpr int PH_WAVES = 8;  // wavelengths per phasor thread
+struct PhasorOp {
+  const void* amp;
+  const void* opd;
+  int amp_dtype, opd_dtype;
+  long long ld_amp, ld_opd;
+  int m, n, trans, nwave;
+  const PhasorWave* waves;
+  void* dst;
+  long long wave_stride, rows_pad;
+  int kpad, pad_;
+  long long begin;
+};
+__host__ __device__ inline long long phasor_threads(long long rows_pad, int kpad, int nwave) {
+  return rows_pad * (kpad / 16) * ((nwave + PH_WAVES - 1) / PH_WAVES);
+}
+
 // launchers (mftx_gemm.hip, mftx_kernels.hip)
 // bn: output columns per workgroup (64 or 128, see valid_bn).
 hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, const Problem* probs,
@@ -104,10 +124,8 @@ hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, c
 hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n,
                        long long lds, int trans, double fold, double x0, void* dst, long long rows_pad, int kpad,
                        hipStream_t s);
-hipError_t launch_phasor(int prec, const void* amp, int amp_dtype, long long ld_amp,
-                         const void* opd, int opd_dtype, long long ld_opd, int m, int n, int trans,
-                         const PhasorWave* waves_dev, int nwave, void* dst,
-                         long long dst_wave_stride, long long rows_pad, int kpad, hipStream_t s);
+// every op of a plan in one launch (ops_dev on the device, sorted by begin)
+hipError_t launch_phasor_group(int prec, const PhasorOp* ops_dev, int nops, long long total_threads, hipStream_t s);
 hipError_t launch_phasor_field(const void* amp, int amp_dtype, long long ld_amp, const void* opd,
                                int opd_dtype, long long ld_opd, int m, int n, double inv_wavelength,
                                void* dst, hipStream_t s);

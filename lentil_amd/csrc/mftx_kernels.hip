@@ -91,43 +91,45 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
   return dtype == 3 ? ((const double*)p)[idx] : (double)((const float*)p)[idx];
 }
 
-// One thread per (row, 16-wide K block, group of PH_WAVES wavelengths):
-// the 16 amp/opd values are read once and the packed operand of the m x n
-// tile (n x m when trans) is written for each wavelength of the group, with
-// that wavelength's stage-1 fold along K folded into the same phase:
-//   amp * cis(opd * inv_wl + fold * (x0 + k)).
-// Consecutive threads take consecutive source columns (K blocks when not
-// trans, rows of the transposed operand when trans), so the source reads of
-// a wave are contiguous.
-constexpr int PH_WAVES = 8;
-
-__global__ void phasor_kernel(int prec, const void* __restrict__ amp, int amp_dtype, long long ld_amp,
-                              const void* __restrict__ opd, int opd_dtype, long long ld_opd, int m, int n,
-                              int trans, const PhasorWave* __restrict__ waves, int nwave, void* __restrict__ dst,
-                              long long wave_stride, long long rows_pad, int kpad) {
-  const int kblocks = kpad / 16;
+// One thread per (op, row, 16-wide K block, group of PH_WAVES wavelengths),
+// every phasor operand of a plan in one launch (the op is found by binary
+// search on its first thread).  The 16 amp/opd values are read once and the
+// packed operand of the m x n tile (n x m when trans) is written for each
+// wavelength of the group, with that wavelength's stage-1 fold along K in the
+// same phase:  amp * cis(opd * inv_wl + fold * (x0 + k)).  Consecutive threads
+// take consecutive source columns (K blocks when not trans, rows of the
+// transposed operand when trans), so the source reads of a wave are contiguous.
+__global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, int nops, long long total) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long per_group = rows_pad * kblocks;
-  const int ngroups = (nwave + PH_WAVES - 1) / PH_WAVES;
-  if (t >= per_group * ngroups) return;
-  const int g = (int)(t / per_group);
-  const long long r = t % per_group;
-  const long long row = trans ? r % rows_pad : r / kblocks;
-  const int kb = (int)(trans ? r / rows_pad : r % kblocks);
+  if (t >= total) return;
+  int lo = 0, hi = nops - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ops[mid].begin <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const PhasorOp op = ops[lo];
+  const long long u = t - op.begin;
+  const int kblocks = op.kpad / 16;
+  const long long per_group = op.rows_pad * kblocks;
+  const int g = (int)(u / per_group);
+  const long long r = u % per_group;
+  const long long row = op.trans ? r % op.rows_pad : r / kblocks;
+  const int kb = (int)(op.trans ? r / op.rows_pad : r % kblocks);
   float a[16];
   double o[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const long long k = (long long)kb * 16 + i;
-    const long long sr = trans ? k : row, sc = trans ? row : k;
-    const bool in = sr < m && sc < n;
-    a[i] = in ? (float)load_real(amp, amp_dtype, sr * ld_amp + sc) : 0.f;
-    o[i] = in ? load_real(opd, opd_dtype, sr * ld_opd + sc) : 0.0;
+    const long long sr = op.trans ? k : row, sc = op.trans ? row : k;
+    const bool in = sr < op.m && sc < op.n;
+    a[i] = in ? (float)load_real(op.amp, op.amp_dtype, sr * op.ld_amp + sc) : 0.f;
+    o[i] = in ? load_real(op.opd, op.opd_dtype, sr * op.ld_opd + sc) : 0.0;
   }
-  const long long lda = packed_lda(prec, kpad);
-  const int w_end = min(nwave, (g + 1) * PH_WAVES);
+  const long long lda = packed_lda(prec, op.kpad);
+  const int w_end = min(op.nwave, (g + 1) * PH_WAVES);
   for (int w = g * PH_WAVES; w < w_end; ++w) {
-    const PhasorWave pw = waves[w];
+    const PhasorWave pw = op.waves[w];
     const double f0 = pw.fold * (pw.x0 + (double)kb * 16);
     float2 v[16];
 #pragma unroll
@@ -135,19 +137,15 @@ __global__ void phasor_kernel(int prec, const void* __restrict__ amp, int amp_dt
       const float2 c = cis_turns(__builtin_fma(o[i], pw.inv_wl, __builtin_fma(pw.fold, (double)i, f0)));
       v[i] = make_float2(a[i] * c.x, a[i] * c.y);
     }
-    store_packed16(prec, (char*)dst + (long long)w * wave_stride, lda, row, kb, v);
+    store_packed16(prec, (char*)op.dst + (long long)w * op.wave_stride, lda, row, kb, v);
   }
 }
 
-hipError_t launch_phasor(int prec, const void* amp, int amp_dtype, long long ld_amp, const void* opd,
-                         int opd_dtype, long long ld_opd, int m, int n, int trans, const PhasorWave* waves, int nwave,
-                         void* dst, long long wave_stride, long long rows_pad, int kpad, hipStream_t s) {
-  const long long total = rows_pad * (kpad / 16) * ((nwave + PH_WAVES - 1) / PH_WAVES);
-  if (total <= 0) return hipSuccess;
+hipError_t launch_phasor_group(int prec, const PhasorOp* ops, int nops, long long total, hipStream_t s) {
+  if (total <= 0 || nops <= 0) return hipSuccess;
   const int bs = 256;
-  hipLaunchKernelGGL(phasor_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, amp,
-                     amp_dtype, ld_amp, opd, opd_dtype, ld_opd, m, n, trans, waves, nwave, dst, wave_stride,
-                     rows_pad, kpad);
+  hipLaunchKernelGGL(phasor_group_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, ops, nops,
+                     total);
   return hipGetLastError();
 }
 
