@@ -43,6 +43,9 @@ def parse():
                     help='output columns per GEMM workgroup (0: chosen by the autotuner)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the FP64 error check')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain per-wavelength FP32 path')
+    ap.add_argument('--workload', default='broadband_512',
+                    choices=['broadband_512', 'segmented_18', 'stress_2048', 'phase_retrieval_512'],
+                    help='broadband_512 is the headline line; the others measure the remaining configurations')
     ap.add_argument('--copy-mib', type=int, default=0,
                     help='after the timed steps, run one plain device copy of this many MiB (a known-size '
                          'kernel that scales hardware byte counters)')
@@ -99,6 +102,110 @@ def cpu_baseline(cfg, budget_s=25.0):
     return len(wl) / med, img, threads, sample
 
 
+def _segmented_pupil(L, cfg):
+    return L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], mask=cfg['mask'], pixelscale=cfg['dx'],
+                   focal_length=cfg['focal_length'])
+
+
+def bench_segmented(args, name):
+    """segmented_18 / stress_2048: one step = the full multi-tile broadband
+    image; a PSF is one wavelength's coherent image of all tiles."""
+    import torch
+    import lentil_amd as L
+    from lentil_amd import workloads
+    from oracle import optics
+    cfg = getattr(workloads, name)()
+    wl, wt = cfg['wavelengths'], cfg['weights']
+    plan = L.PSFPlan(_segmented_pupil(L, cfg), wl, wt, pixelscale=cfg['pixelscale'], shape=cfg['shape'],
+                     oversample=cfg['oversample'], tilts=[L.Tilt(*t) for t in cfg['tilts']],
+                     precision=args.precision, autotune=True)
+    out = torch.empty(plan.shape_out, dtype=torch.float32, device='cuda')
+    for _ in range(args.warmup):
+        plan(out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan(out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ms = plan.profile(torch.empty_like(out), repeats=3)
+    # error against the oracle on the two extreme wavelengths (bounded host work)
+    idx = np.array([0, len(wl) - 1])
+    sub = L.PSFPlan(_segmented_pupil(L, cfg), wl[idx], wt[idx], pixelscale=cfg['pixelscale'], shape=cfg['shape'],
+                    oversample=cfg['oversample'], tilts=[L.Tilt(*t) for t in cfg['tilts']], precision=args.precision,
+                    lambda_group=plan.lambda_group, tile_n=plan.tile_n)
+    img = sub().double().cpu().numpy()
+    p = optics.Pupil(cfg['amp'], cfg['opd'], cfg['dx'], cfg['focal_length'], mask=cfg['mask'])
+    t1 = time.perf_counter()
+    ref = optics.broadband_image(p, wl[idx], wt[idx], cfg['pixelscale'], cfg['shape'], cfg['oversample'],
+                                 tilts=[optics.Tilt(*t) for t in cfg['tilts']])
+    cpu_s = (time.perf_counter() - t1) / len(idx)
+    err = float(np.linalg.norm(img - ref) / np.linalg.norm(ref))
+    flops2 = plan.info['gemm_flops'][1]
+    line = {
+        'metric': f'PSFs/sec ({name})', 'value': len(wl) / dt, 'unit': 'PSFs/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': 1e3 * dt, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': args.precision, 'data': 'synthetic (seeded segmented pupil, Zernike OPD)',
+        'config': {'workload': name, 'tiles': len(plan.tile_shapes), 'wavelengths': len(wl),
+                   'mfts_per_step': plan.n_problems, 'detector': list(plan.shape_out),
+                   'lambda_group': plan.lambda_group, 'tile_n': plan.tile_n, 'parallelism': 'single GPU'},
+        'mfts_per_s': plan.n_problems / dt, 'tflops_algorithmic': plan.info['flops'] / dt / 1e12,
+        'stage_ms': ms,
+        'roofline': {'bound': 'mfma', 'achieved': flops2 / (ms['stage2'] * 1e-3) / 1e12,
+                     'peak': PEAK[args.precision] / 1e12, 'unit': 'TFLOP/s',
+                     'frac': flops2 / (ms['stage2'] * 1e-3) / PEAK[args.precision], 'traffic': None,
+                     'kernel': 'stage-2 GEMM',
+                     'mfma_pipe_frac': plan.info['mfma_flops'][1] / (ms['stage2'] * 1e-3) / PEAK[args.precision]},
+        'rel_l2_vs_fp64_two_wavelengths': err,
+        'cpu_baseline': {'value': 1.0 / cpu_s, 'unit': 'PSFs/s', 'cores': os.cpu_count(), 'kind': 'port',
+                         'sample': f'complex128 oracle, wavelengths {wl[0]:.3g} and {wl[-1]:.3g} m'},
+        'autotune_ms_per_image': {f'lambda_group={lg},tile_n={tn}': round(t, 3)
+                                  for (lg, tn), t in sorted(getattr(plan, 'tuning', {}).items())},
+    }
+    print(json.dumps(line))
+
+
+def bench_retrieval(args):
+    """phase_retrieval_512: one step = loss and OPD gradient of 4 diversity
+    images x 16 wavelengths (forward plans + adjoint MFTs recomputing the
+    fields)."""
+    import torch
+    from lentil_amd import workloads
+    from lentil_amd.autograd import BroadbandPSFModel
+    cfg = workloads.phase_retrieval_512()
+    model = BroadbandPSFModel(cfg['amp'], cfg['diversity'], cfg['wavelengths'], cfg['weights'], cfg['dx'],
+                              cfg['pixelscale'], cfg['focal_length'], cfg['shape'], cfg['oversample'],
+                              precision=args.precision)
+    x = torch.tensor(cfg['opd'], device='cuda', requires_grad=True)
+    target = model(x.detach()).detach() * 1.01
+
+    def step():
+        x.grad = None
+        loss = torch.sum((model(x) - target) ** 2)
+        loss.backward()
+        return loss
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    npsf = len(cfg['diversity']) * len(cfg['wavelengths'])
+    line = {
+        'metric': 'loss+gradient iterations/sec (phase_retrieval_512)', 'value': 1.0 / dt, 'unit': 'iterations/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': 1e3 * dt, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': args.precision,
+        'data': 'synthetic (seeded pupil, Zernike OPD, defocus diversity)',
+        'config': {'workload': 'phase_retrieval_512', 'diversities': len(cfg['diversity']),
+                   'wavelengths': len(cfg['wavelengths']), 'pupil': 512, 'detector': 512,
+                   'parallelism': 'single GPU'},
+        'forward_psfs_per_s': npsf / dt,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -108,6 +215,10 @@ def main():
     import torch
     import lentil_amd as L
     from lentil_amd import workloads
+    if args.workload in ('segmented_18', 'stress_2048'):
+        return bench_segmented(args, args.workload)
+    if args.workload == 'phase_retrieval_512':
+        return bench_retrieval(args)
 
     assert torch.cuda.is_available(), 'bench needs a HIP device'
     cfg = workloads.broadband_512()
