@@ -128,6 +128,42 @@ int mftx_phase_grad(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_am
                     int64_t ld_opd, int64_t m, int64_t n, double wavelength, const void* D, int64_t ldD,
                     double* grad_opd, int64_t ld_gopd, double* grad_amp, int64_t ld_gamp);
 
+/* mftx_phase_grad summed over nwave wavelengths in one launch: D holds one
+ * m x n complex64 cotangent per wavelength, D_w at D + w * strideD elements,
+ * and wavelengths[w] (host memory) is its wavelength.  No reference
+ * counterpart (the gradient of a broadband image for phase retrieval). */
+int mftx_phase_grad_many(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd,
+                         int opd_dtype, int64_t ld_opd, int64_t m, int64_t n, const double* wavelengths,
+                         int32_t nwave, const void* D, int64_t ldD, int64_t strideD, double* grad_opd,
+                         int64_t ld_gopd, double* grad_amp, int64_t ld_gamp);
+
+/* ---------------------------------------------------------------------------
+ * Many independent MFTs in one grouped call (no reference counterpart: the
+ * reference runs one fourier.dft2 per call).  Each entry is one mftx_dft2
+ * (same arguments and meaning), with two additions:
+ *   - when f is NULL the input is the pupil phasor
+ *     amp * exp(2*pi*i*opd*inv_wl) (plane.py:503-510) generated in place;
+ *   - when w is not NULL the output is multiplied elementwise by the real
+ *     M x N map w (float32/float64, row stride ldw): the weighted cotangent
+ *     of an intensity for the adjoint pass of phase retrieval.
+ * All entries run in one pack launch and one launch per GEMM stage.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  const void* f;       /* input m x n (f_dtype, row stride ldf), or NULL for a phasor */
+  const void* amp;     /* phasor amplitude and OPD (dtype 1 or 3), used when f is NULL */
+  const void* opd;
+  int32_t f_dtype, amp_dtype, opd_dtype, conj;
+  int64_t m, n, ldf, ld_amp, ld_opd;
+  double inv_wl;       /* 1 / wavelength of the phasor */
+  void* F;             /* output M x N complex64, row stride ldF */
+  int64_t M, N, ldF;
+  double alpha_r, alpha_c, shift_r, shift_c, offset_r, offset_c, scale;
+  const void* w;       /* optional real weight map */
+  int32_t w_dtype, pad_;
+  int64_t ldw;
+} mftx_mft;
+int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision);
+
 /* Fill `bytes` bytes at device pointer p with zeros (a kernel, not a memset node). */
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes);
 

@@ -48,6 +48,15 @@ class MftxPlanInfo(ctypes.Structure):
                 ('n_work1', ctypes.c_int32), ('n_work2', ctypes.c_int32)]
 
 
+class MftxMft(ctypes.Structure):
+    _fields_ = [('f', c_vp), ('amp', c_vp), ('opd', c_vp), ('f_dtype', ctypes.c_int32), ('amp_dtype', ctypes.c_int32),
+                ('opd_dtype', ctypes.c_int32), ('conj', ctypes.c_int32), ('m', c_i64), ('n', c_i64), ('ldf', c_i64),
+                ('ld_amp', c_i64), ('ld_opd', c_i64), ('inv_wl', c_dbl), ('F', c_vp), ('M', c_i64), ('N', c_i64),
+                ('ldF', c_i64), ('alpha_r', c_dbl), ('alpha_c', c_dbl), ('shift_r', c_dbl), ('shift_c', c_dbl),
+                ('offset_r', c_dbl), ('offset_c', c_dbl), ('scale', c_dbl), ('w', c_vp), ('w_dtype', ctypes.c_int32),
+                ('pad_', ctypes.c_int32), ('ldw', c_i64)]
+
+
 # name -> (restype, argtypes); exactly the exports of include/mftx.h
 SIGNATURES = {
     'mftx_version': (c_int, []),
@@ -62,6 +71,9 @@ SIGNATURES = {
     'mftx_insert': (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_int, c_i64,
                             c_i64, c_i64, c_int, c_dbl]),
     'mftx_multiply': (c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64]),
+    'mftx_dft2_many': (c_int, [c_vp, ctypes.POINTER(MftxMft), ctypes.c_int32, ctypes.c_int32]),
+    'mftx_phase_grad_many': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, c_i64, c_vp,
+                                     ctypes.c_int32, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64]),
     'mftx_zero': (c_int, [c_vp, c_vp, c_i64]),
     'mftx_weight_field': (c_int, [c_vp, c_vp, c_i64, c_vp, c_int, c_i64, c_dbl, c_vp, c_i64, c_i64, c_i64]),
     'mftx_phase_grad': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, c_i64, c_dbl, c_vp, c_i64,

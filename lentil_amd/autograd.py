@@ -14,9 +14,9 @@ fields (nothing is stored but the inputs) and applies the adjoint MFT:
     D = dL/d conj(f) = dft2_adjoint(w g F),   g = dL/dI
     dL/d opd = sum (2 pi / l) (-2) Im(conj(D) f),  dL/d amp = sum 2 Re(conj(D) e^{i theta})
 
-The forward runs as one batched PSFPlan per diversity; the backward uses
-mftx_dft2 (forward and conjugate twiddles), mftx_weight_field and
-mftx_phase_grad.
+The forward runs as one batched PSFPlan per diversity; the backward is two
+grouped MFT calls (mftx_dft2_many: the weighted forward fields, then the
+adjoint with conjugate twiddles) and one mftx_phase_grad_many per diversity.
 """
 import ctypes
 
@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from .batched import PSFPlan
-from .plane import Pupil, device_phasor
+from .plane import Pupil
 
 
 def _ptr(t):
@@ -73,35 +73,53 @@ class BroadbandPSFModel:
 
     # --------------------------------------------------------------- backward
     def backward_grads(self, opd, amp, grad_images, need_amp):
+        """Gradients of sum_d <g_d, I_d> with respect to opd (and amp).
+
+        Two grouped MFT calls cover every (diversity, wavelength) pair: the
+        forward fields recomputed from the phasor with the weighted cotangent
+        w_l * g_d folded into the store, Y = w_l g_d F, then the adjoint MFT
+        D = dft2_adjoint(Y); one phase-gradient launch per diversity sums the
+        wavelengths."""
         lib = _lib.load()
         ctx = _lib.context()
         m, n = self.shape
         M = self.M
+        nd, nw = len(self.div), len(self.wavelengths)
         prec = _lib.precision_code(self.precision)
         amp64 = amp.detach().to(torch.float64).contiguous()
         g_opd = torch.zeros((m, n), dtype=torch.float64, device=self.dev)
         g_amp = torch.zeros((m, n), dtype=torch.float64, device=self.dev) if need_amp else None
-        F = torch.empty((M, M), dtype=torch.complex64, device=self.dev)
-        Y = torch.empty_like(F)
-        D = torch.empty((m, n), dtype=torch.complex64, device=self.dev)
+        Y = torch.empty((nd * nw, M, M), dtype=torch.complex64, device=self.dev)
+        D = torch.empty((nd * nw, m, n), dtype=torch.complex64, device=self.dev)
         gI = grad_images.detach().contiguous()
-        for d, div in enumerate(self.div):
-            tot = (opd.detach() + div).contiguous()
-            for wl, wt, a in zip(self.wavelengths, self.weights, self.alphas):
-                scale = float(np.sqrt(a * a))
-                f = device_phasor(amp64, tot, wl)                       # recompute the field
-                _lib.check(lib.mftx_dft2(ctx, _ptr(f), _lib.DT_C64, m, n, n, _ptr(F), M, M, M, a, a, 0.0, 0.0,
-                                         0.0, 0.0, scale, 0, prec), ctx, 'mftx_dft2')
-                gd = gI[d]
-                _lib.check(lib.mftx_weight_field(ctx, _ptr(F), M, _ptr(gd), _lib.dtype_code(gd), gd.stride(0),
-                                                 float(wt), _ptr(Y), M, M, M), ctx, 'mftx_weight_field')
-                # adjoint MFT: conjugate twiddles from the M x M grid back to m x n
-                _lib.check(lib.mftx_dft2(ctx, _ptr(Y), _lib.DT_C64, M, M, M, _ptr(D), m, n, n, a, a, 0.0, 0.0,
-                                         0.0, 0.0, scale, 1, prec), ctx, 'mftx_dft2 (adjoint)')
-                _lib.check(lib.mftx_phase_grad(ctx, _ptr(amp64), _lib.DT_F64, n, _ptr(tot), _lib.DT_F64, n, m, n,
-                                               float(wl), _ptr(D), n, _ptr(g_opd), n,
-                                               _ptr(g_amp) if need_amp else None, n),
-                           ctx, 'mftx_phase_grad')
+        tots = [(opd.detach() + div).contiguous() for div in self.div]
+        fwd = (_lib.MftxMft * (nd * nw))()
+        adj = (_lib.MftxMft * (nd * nw))()
+        for d in range(nd):
+            for j, (wl, wt, a) in enumerate(zip(self.wavelengths, self.weights, self.alphas)):
+                i = d * nw + j
+                unit = float(abs(a))                      # sqrt(|a * a|), the unitary scale
+                q = fwd[i]
+                q.amp, q.amp_dtype, q.ld_amp = amp64.data_ptr(), _lib.DT_F64, n
+                q.opd, q.opd_dtype, q.ld_opd = tots[d].data_ptr(), _lib.DT_F64, n
+                q.m, q.n, q.inv_wl = m, n, 1.0 / float(wl)
+                q.F, q.M, q.N, q.ldF = Y[i].data_ptr(), M, M, M
+                q.alpha_r = q.alpha_c = a
+                q.scale = unit * float(wt)
+                q.w, q.w_dtype, q.ldw = gI[d].data_ptr(), _lib.dtype_code(gI[d]), gI.stride(1)
+                r = adj[i]
+                r.f, r.f_dtype, r.m, r.n, r.ldf = Y[i].data_ptr(), _lib.DT_C64, M, M, M
+                r.F, r.M, r.N, r.ldF = D[i].data_ptr(), m, n, n
+                r.alpha_r = r.alpha_c = a
+                r.scale, r.conj = unit, 1
+        _lib.check(lib.mftx_dft2_many(ctx, fwd, nd * nw, prec), ctx, 'mftx_dft2_many (weighted forward)')
+        _lib.check(lib.mftx_dft2_many(ctx, adj, nd * nw, prec), ctx, 'mftx_dft2_many (adjoint)')
+        wls = (ctypes.c_double * nw)(*[float(w) for w in self.wavelengths])
+        for d in range(nd):
+            _lib.check(lib.mftx_phase_grad_many(ctx, _ptr(amp64), _lib.DT_F64, n, _ptr(tots[d]), _lib.DT_F64, n, m, n,
+                                                wls, nw, _ptr(D[d * nw]), n, m * n, _ptr(g_opd), n,
+                                                _ptr(g_amp) if need_amp else None, n),
+                       ctx, 'mftx_phase_grad_many')
         return g_opd, g_amp
 
 

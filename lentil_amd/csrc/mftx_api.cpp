@@ -203,19 +203,40 @@ static int ensure_ws(mftx_ctx* ctx, size_t bytes) {
   return MFTX_OK;
 }
 
+// Take the next ring slot (waiting until its previous upload has been
+// consumed); *host is its pinned staging area, *dev its device address.
+static int reserve_desc(mftx_ctx* ctx, int* slot, char** host, char** dev) {
+  const int s = ctx->ring_next;
+  ctx->ring_next = (s + 1) % kRingSlots;
+  if (ctx->ring_used[s]) HIPCHK(ctx, hipEventSynchronize(ctx->ring_ev[s]));
+  ctx->ring_used[s] = false;
+  *slot = s;
+  *host = ctx->ring_host + (size_t)s * kSlotBytes;
+  *dev = ctx->ring_dev + (size_t)s * kSlotBytes;
+  return MFTX_OK;
+}
+
+// Queue the upload of the first `bytes` of a reserved slot on the context stream.
+static int commit_desc(mftx_ctx* ctx, int s, size_t bytes) {
+  char* h = ctx->ring_host + (size_t)s * kSlotBytes;
+  char* d = ctx->ring_dev + (size_t)s * kSlotBytes;
+  HIPCHK(ctx, hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ring_ev[s], ctx->stream));
+  ctx->ring_used[s] = true;
+  return MFTX_OK;
+}
+
 // Stage `bytes` of descriptors into a ring slot and queue the upload on the
 // context stream; returns the device address of the slot.
 static int upload_desc(mftx_ctx* ctx, const void* src, size_t bytes, void** dev) {
   if (bytes > kSlotBytes) return fail(ctx, MFTX_EINVAL, "descriptor block too large (output too big for one call)");
-  const int s = ctx->ring_next;
-  ctx->ring_next = (s + 1) % kRingSlots;
-  if (ctx->ring_used[s]) HIPCHK(ctx, hipEventSynchronize(ctx->ring_ev[s]));
-  char* h = ctx->ring_host + (size_t)s * kSlotBytes;
-  char* d = ctx->ring_dev + (size_t)s * kSlotBytes;
+  int s;
+  char *h, *d;
+  int rc = reserve_desc(ctx, &s, &h, &d);
+  if (rc) return rc;
   std::memcpy(h, src, bytes);
-  HIPCHK(ctx, hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipEventRecord(ctx->ring_ev[s], ctx->stream));
-  ctx->ring_used[s] = true;
+  rc = commit_desc(ctx, s, bytes);
+  if (rc) return rc;
   *dev = d;
   return MFTX_OK;
 }
@@ -241,6 +262,232 @@ static int run_single(mftx_ctx* ctx, int prec, int epi, int bn, Problem p, doubl
   char* d = (char*)dev;
   HIPCHK(ctx, launch_gemm(prec, epi, bn, (const Work*)(d + off_w), (int)works.size(), (const Problem*)d,
                           (const double*)(d + off_cx), ctx->stream));
+  return MFTX_OK;
+}
+
+// Geometry of one entry of mftx_dft2_many (the quantities mftx_dft2 derives).
+namespace {
+struct ManyGeo {
+  bool col_first;
+  int64_t r1, c1, rows1, rows2;
+  double alpha1, alpha2, x01, x02;
+  OutGrid g1, g2;
+  int kpad1, kpad2;
+  size_t off1, off2;  // workspace offsets of the packed stage-1 and stage-2 operands
+};
+}  // namespace
+
+// All entries [0, count) in one pack launch per input kind and one GEMM launch
+// per stage.  A batch whose descriptors exceed one ring slot is split in two.
+static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int prec) {
+  std::vector<ManyGeo> geo(count);
+  int64_t wide1 = 0, wide2 = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const mftx_mft& q = mfts[i];
+    ManyGeo& g = geo[i];
+    g.col_first = cost_col_first(q.m, q.n, q.M, q.N) < cost_row_first(q.m, q.n, q.M, q.N);
+    g.r1 = g.col_first ? q.n : q.m;
+    g.c1 = g.col_first ? q.M : q.N;
+    const int64_t c2 = g.col_first ? q.N : q.M;
+    g.rows1 = round_up(g.r1, BM);
+    g.rows2 = round_up(g.c1, BM);
+    const double sgn = q.conj ? 1.0 : -1.0;
+    g.alpha1 = sgn * (g.col_first ? q.alpha_r : q.alpha_c);
+    g.alpha2 = sgn * (g.col_first ? q.alpha_c : q.alpha_r);
+    g.g1 = make_grid(g.alpha1, g.col_first ? q.shift_r : q.shift_c, (int)g.c1);
+    g.g2 = make_grid(g.alpha2, g.col_first ? q.shift_c : q.shift_r, (int)c2);
+    const double xr0 = -centre(q.m) + q.offset_r, xc0 = -centre(q.n) + q.offset_c;
+    g.x01 = g.col_first ? xr0 : xc0;
+    g.x02 = g.col_first ? xc0 : xr0;
+    wide1 += (g.rows1 / BM) * ((g.g1.slots + 63) / 64);
+    wide2 += (g.rows2 / BM) * ((g.g2.slots + 63) / 64);
+  }
+  // 128-column tiles when the larger stage still gives every CU a workgroup
+  const int bn = std::max(wide1, wide2) >= 256 ? 128 : 64;
+  const int kg = k_granule(bn);
+  size_t ws = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const mftx_mft& q = mfts[i];
+    ManyGeo& g = geo[i];
+    g.kpad1 = (int)round_up(g.col_first ? q.m : q.n, kg);
+    g.kpad2 = (int)round_up(g.r1, kg);
+    g.off1 = ws;
+    ws += round_up((int64_t)packed_bytes(prec, g.rows1, g.kpad1), 256);
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    geo[i].off2 = ws;
+    ws += round_up((int64_t)packed_bytes(prec, geo[i].rows2, geo[i].kpad2), 256);
+  }
+
+  // ---- descriptors -------------------------------------------------------------
+  std::vector<PhasorWave> waves;
+  std::vector<PhasorOp> phops;
+  std::vector<PackOp> packs;
+  long long ph_total = 0, pk_total = 0;
+  std::vector<int> wave_of;  // phasor wave index of entry i (-1 for complex input)
+  for (int32_t i = 0; i < count; ++i) {
+    const mftx_mft& q = mfts[i];
+    const ManyGeo& g = geo[i];
+    if (q.f) {
+      PackOp op = {};
+      op.src = q.f;
+      op.dtype = q.f_dtype;
+      op.trans = g.col_first ? 1 : 0;
+      op.m = q.m;
+      op.n = q.n;
+      op.lds = q.ldf;
+      op.fold = g.g1.fold;
+      op.x0 = g.x01;
+      op.dst = nullptr;  // rebased below
+      op.rows_pad = g.rows1;
+      op.kpad = g.kpad1;
+      op.begin = pk_total;
+      pk_total += g.rows1 * (g.kpad1 / 16);
+      packs.push_back(op);
+      wave_of.push_back(-(int)packs.size());  // -(pack index + 1)
+      continue;
+    }
+    // consecutive phasor entries of one pupil and geometry share an op
+    // (their operands are one fixed stride apart in the workspace)
+    const size_t stride = round_up((int64_t)packed_bytes(prec, g.rows1, g.kpad1), 256);
+    bool joined = false;
+    if (i > 0 && !mfts[i - 1].f && !phops.empty()) {
+      const mftx_mft& p = mfts[i - 1];
+      const ManyGeo& h = geo[i - 1];
+      PhasorOp& last = phops.back();
+      joined = p.amp == q.amp && p.opd == q.opd && p.amp_dtype == q.amp_dtype && p.opd_dtype == q.opd_dtype &&
+               p.ld_amp == q.ld_amp && p.ld_opd == q.ld_opd && p.m == q.m && p.n == q.n &&
+               h.col_first == g.col_first && h.kpad1 == g.kpad1 && h.rows1 == g.rows1 &&
+               (long long)stride == last.wave_stride;
+    }
+    if (!joined) {
+      if (!phops.empty()) ph_total += phasor_threads(phops.back().rows_pad, phops.back().kpad, phops.back().nwave);
+      PhasorOp op = {};
+      op.amp = q.amp;
+      op.opd = q.opd;
+      op.amp_dtype = q.amp_dtype;
+      op.opd_dtype = q.opd_dtype;
+      op.ld_amp = q.ld_amp;
+      op.ld_opd = q.ld_opd;
+      op.m = (int)q.m;
+      op.n = (int)q.n;
+      op.trans = g.col_first ? 1 : 0;
+      op.nwave = 0;
+      op.waves = (const PhasorWave*)(uintptr_t)waves.size();  // index, rebased below
+      op.dst = (void*)(uintptr_t)g.off1;                       // offset, rebased below
+      op.wave_stride = (long long)stride;
+      op.rows_pad = g.rows1;
+      op.kpad = g.kpad1;
+      op.begin = ph_total;
+      phops.push_back(op);
+    }
+    phops.back().nwave += 1;
+    waves.push_back(PhasorWave{q.inv_wl, g.g1.fold, g.x01});
+    wave_of.push_back((int)waves.size() - 1);
+  }
+  if (!phops.empty()) ph_total += phasor_threads(phops.back().rows_pad, phops.back().kpad, phops.back().nwave);
+
+  std::vector<Problem> p1(count), p2(count);
+  std::vector<double> cx;
+  std::vector<std::vector<Work>> clus1, clus2;
+  for (int32_t i = 0; i < count; ++i) {
+    const mftx_mft& q = mfts[i];
+    const ManyGeo& g = geo[i];
+    Problem& a = p1[i];
+    a.lda = packed_lda(prec, g.kpad1);
+    a.alpha = g.alpha1;
+    a.y0 = g.g1.y0;
+    a.rows = (int)g.r1;
+    a.cols = g.g1.slots;
+    a.ncols = (int)g.c1;
+    a.cp = g.g1.cp;
+    a.cm = g.g1.cm;
+    a.kpad = g.kpad1;
+    a.chunk_begin = (int)cx.size();
+    push_chunks(cx, g.kpad1, g.x01);
+    a.scale = 1.0f;
+    a.limit = g.kpad2;
+    a.ldd = packed_lda(prec, g.kpad2);
+    a.fold = g.g2.fold;
+    a.fold_x0 = g.x02;
+    add_tiles(clus1, i, i + 1, g.rows1, a.cols, bn);
+
+    Problem& b = p2[i];
+    b.lda = packed_lda(prec, g.kpad2);
+    b.alpha = g.alpha2;
+    b.y0 = g.g2.y0;
+    b.rows = (int)g.c1;
+    b.cols = g.g2.slots;
+    b.ncols = (int)(g.col_first ? q.N : q.M);
+    b.cp = g.g2.cp;
+    b.cm = g.g2.cm;
+    b.kpad = g.kpad2;
+    b.chunk_begin = (int)cx.size();
+    push_chunks(cx, g.kpad2, g.x02);
+    b.scale = (float)q.scale;
+    b.dst = q.F;
+    b.ldd = q.ldF;
+    b.trans = g.col_first ? 0 : 1;
+    b.wmap = q.w;
+    b.ldw = q.ldw;
+    b.wdtype = q.w_dtype;
+    add_tiles(clus2, i, i + 1, g.rows2, b.cols, bn);
+  }
+  const std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
+
+  // one descriptor block: waves, phasor ops, pack ops, problems, chunk origins, works
+  size_t off = 0;
+  auto take = [&](size_t b) { const size_t o = off; off = round_up((int64_t)(off + b), 16); return o; };
+  const size_t o_wv = take(sizeof(PhasorWave) * waves.size());
+  const size_t o_ph = take(sizeof(PhasorOp) * phops.size());
+  const size_t o_pk = take(sizeof(PackOp) * packs.size());
+  const size_t o_p1 = take(sizeof(Problem) * p1.size());
+  const size_t o_p2 = take(sizeof(Problem) * p2.size());
+  const size_t o_cx = take(sizeof(double) * cx.size());
+  const size_t o_w1 = take(sizeof(Work) * w1.size());
+  const size_t o_w2 = take(sizeof(Work) * w2.size());
+  if (off > kSlotBytes) {
+    if (count == 1) return fail(ctx, MFTX_EINVAL, "descriptor block too large (output too big for one call)");
+    const int32_t h = count / 2;
+    const int rc = dft2_many_run(ctx, mfts, h, prec);
+    return rc ? rc : dft2_many_run(ctx, mfts + h, count - h, prec);
+  }
+  int rc = ensure_ws(ctx, ws);
+  if (rc) return rc;
+  char* base = (char*)ctx->ws;
+  // the slot is reserved first: the descriptors hold device addresses inside it
+  int slot;
+  char *h, *d;
+  rc = reserve_desc(ctx, &slot, &h, &d);
+  if (rc) return rc;
+  for (auto& op : phops) {
+    op.waves = (const PhasorWave*)(d + o_wv) + (uintptr_t)op.waves;
+    op.dst = base + (uintptr_t)op.dst;
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    p1[i].a = base + geo[i].off1;
+    p1[i].dst = base + geo[i].off2;
+    p2[i].a = base + geo[i].off2;
+    if (wave_of[i] < 0) packs[-wave_of[i] - 1].dst = base + geo[i].off1;
+  }
+  std::memcpy(h + o_wv, waves.data(), sizeof(PhasorWave) * waves.size());
+  std::memcpy(h + o_ph, phops.data(), sizeof(PhasorOp) * phops.size());
+  std::memcpy(h + o_pk, packs.data(), sizeof(PackOp) * packs.size());
+  std::memcpy(h + o_p1, p1.data(), sizeof(Problem) * p1.size());
+  std::memcpy(h + o_p2, p2.data(), sizeof(Problem) * p2.size());
+  std::memcpy(h + o_cx, cx.data(), sizeof(double) * cx.size());
+  std::memcpy(h + o_w1, w1.data(), sizeof(Work) * w1.size());
+  std::memcpy(h + o_w2, w2.data(), sizeof(Work) * w2.size());
+  rc = commit_desc(ctx, slot, off);
+  if (rc) return rc;
+  if (!phops.empty())
+    HIPCHK(ctx, launch_phasor_group(prec, (const PhasorOp*)(d + o_ph), (int)phops.size(), ph_total, ctx->stream));
+  if (!packs.empty())
+    HIPCHK(ctx, launch_pack_group(prec, (const PackOp*)(d + o_pk), (int)packs.size(), pk_total, ctx->stream));
+  HIPCHK(ctx, launch_gemm(prec, EPI_STORE_PACKED_T, bn, (const Work*)(d + o_w1), (int)w1.size(),
+                          (const Problem*)(d + o_p1), (const double*)(d + o_cx), ctx->stream));
+  HIPCHK(ctx, launch_gemm(prec, EPI_STORE_CPLX, bn, (const Work*)(d + o_w2), (int)w2.size(),
+                          (const Problem*)(d + o_p2), (const double*)(d + o_cx), ctx->stream));
   return MFTX_OK;
 }
 
@@ -442,9 +689,70 @@ int mftx_phase_grad(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_am
     return fail(ctx, MFTX_EINVAL, "bad gradient shape");
   if (!(wavelength > 0)) return fail(ctx, MFTX_EINVAL, "wavelength must be positive");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, launch_phase_grad(amp, amp_dtype, ld_amp, opd, opd_dtype, ld_opd, (int)m, (int)n, 1.0 / wavelength, D,
-                                ldD, grad_opd, ld_gopd, grad_amp, ld_gamp, ctx->stream));
+  HIPCHK(ctx, launch_phase_grad(amp, amp_dtype, ld_amp, opd, opd_dtype, ld_opd, (int)m, (int)n, nullptr,
+                                1.0 / wavelength, 1, D, ldD, 0, grad_opd, ld_gopd, grad_amp, ld_gamp, ctx->stream));
   return MFTX_OK;
+}
+
+int mftx_phase_grad_many(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd,
+                         int opd_dtype, int64_t ld_opd, int64_t m, int64_t n, const double* wavelengths,
+                         int32_t nwave, const void* D, int64_t ldD, int64_t strideD, double* grad_opd,
+                         int64_t ld_gopd, double* grad_amp, int64_t ld_gamp) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!amp || !opd || !D || !grad_opd || (nwave > 0 && !wavelengths))
+    return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if ((amp_dtype != 1 && amp_dtype != 3) || (opd_dtype != 1 && opd_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "amp/opd must be float32 or float64");
+  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n || ldD < n || ld_gopd < n || (grad_amp && ld_gamp < n))
+    return fail(ctx, MFTX_EINVAL, "bad gradient shape");
+  if (nwave < 0 || (nwave > 1 && strideD < m * ldD)) return fail(ctx, MFTX_EINVAL, "bad wavelength count or stride");
+  if ((size_t)nwave * sizeof(double) > kSlotBytes) return fail(ctx, MFTX_EINVAL, "too many wavelengths");
+  std::vector<double> inv(nwave);
+  for (int32_t w = 0; w < nwave; ++w) {
+    if (!(wavelengths[w] > 0) || !std::isfinite(wavelengths[w]))
+      return fail(ctx, MFTX_EINVAL, "wavelengths must be positive");
+    inv[w] = 1.0 / wavelengths[w];
+  }
+  if (nwave == 0) return MFTX_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  void* dev = nullptr;
+  const int rc = upload_desc(ctx, inv.data(), sizeof(double) * inv.size(), &dev);
+  if (rc) return rc;
+  HIPCHK(ctx, launch_phase_grad(amp, amp_dtype, ld_amp, opd, opd_dtype, ld_opd, (int)m, (int)n, (const double*)dev,
+                                0.0, nwave, D, ldD, strideD, grad_opd, ld_gopd, grad_amp, ld_gamp, ctx->stream));
+  return MFTX_OK;
+}
+
+int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (count < 0 || (count > 0 && !mfts)) return fail(ctx, MFTX_EINVAL, "bad MFT list");
+  if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3) return fail(ctx, MFTX_EINVAL, "unknown precision");
+  for (int32_t i = 0; i < count; ++i) {
+    const mftx_mft& q = mfts[i];
+    const std::string at = "MFT " + std::to_string(i) + ": ";
+    if (!q.F) return fail(ctx, MFTX_EINVAL, at + "null output pointer");
+    if (q.m <= 0 || q.n <= 0 || q.M <= 0 || q.N <= 0) return fail(ctx, MFTX_EINVAL, at + "shapes must be positive");
+    if (q.m > (1 << 20) || q.n > (1 << 20) || q.M > (1 << 20) || q.N > (1 << 20))
+      return fail(ctx, MFTX_EINVAL, at + "dimension too large");
+    if (q.ldF < q.N) return fail(ctx, MFTX_EINVAL, at + "output row stride smaller than row length");
+    if (q.f) {
+      if (q.f_dtype < 0 || q.f_dtype > 3) return fail(ctx, MFTX_EINVAL, at + "unknown input dtype");
+      if (q.ldf < q.n) return fail(ctx, MFTX_EINVAL, at + "input row stride smaller than row length");
+    } else {
+      if (!q.amp || !q.opd) return fail(ctx, MFTX_EINVAL, at + "phasor input needs amp and opd");
+      if ((q.amp_dtype != 1 && q.amp_dtype != 3) || (q.opd_dtype != 1 && q.opd_dtype != 3))
+        return fail(ctx, MFTX_EINVAL, at + "amp/opd must be float32 or float64");
+      if (q.ld_amp < q.n || q.ld_opd < q.n) return fail(ctx, MFTX_EINVAL, at + "phasor row stride too small");
+      if (!(q.inv_wl > 0) || !std::isfinite(q.inv_wl)) return fail(ctx, MFTX_EINVAL, at + "inv_wl must be positive");
+    }
+    if (q.w) {
+      if (q.w_dtype != 1 && q.w_dtype != 3) return fail(ctx, MFTX_EINVAL, at + "weight must be float32 or float64");
+      if (q.ldw < q.N) return fail(ctx, MFTX_EINVAL, at + "weight row stride smaller than row length");
+    }
+  }
+  if (count == 0) return MFTX_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return dft2_many_run(ctx, mfts, count, precision);
 }
 
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes) {

@@ -291,19 +291,24 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
     }
     if (EPI == EPI_STORE_CPLX) {
       float2* dst = (float2*)P.dst;
+      // optional real weight map, indexed like the destination
+      auto weight = [&](long long idx) -> float {
+        if (!P.wmap) return 1.0f;
+        return P.wdtype == 3 ? (float)((const double*)P.wmap)[idx] : ((const float*)P.wmap)[idx];
+      };
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = row0 + e;
         if (row < P.rows) {
           if (has_p) {
-            const float2 v = make_float2(pre[e], pim[e]);
-            if (P.trans) dst[(long long)jp * P.ldd + row] = v;
-            else dst[(long long)row * P.ldd + jp] = v;
+            const long long o = P.trans ? (long long)jp * P.ldd + row : (long long)row * P.ldd + jp;
+            const float w = weight(P.trans ? (long long)jp * P.ldw + row : (long long)row * P.ldw + jp);
+            dst[o] = make_float2(pre[e] * w, pim[e] * w);
           }
           if (has_m) {
-            const float2 v = make_float2(mre[e], mim[e]);
-            if (P.trans) dst[(long long)jm * P.ldd + row] = v;
-            else dst[(long long)row * P.ldd + jm] = v;
+            const long long o = P.trans ? (long long)jm * P.ldd + row : (long long)row * P.ldd + jm;
+            const float w = weight(P.trans ? (long long)jm * P.ldw + row : (long long)row * P.ldw + jm);
+            dst[o] = make_float2(mre[e] * w, mim[e] * w);
           }
         }
       }

@@ -65,6 +65,10 @@ struct Problem {
   long long ldd;        // destination row stride (elements of its type)
   double fold;          // EPI_STORE_PACKED_T: C row r is stored times cis(fold * (fold_x0 + r)),
   double fold_x0;       //   the next stage's fold
+  const void* wmap;     // EPI_STORE_CPLX: optional real map multiplied into each stored value,
+  long long ldw;        //   indexed like the destination (row stride ldw)
+  int wdtype;           //   1 float32, 3 float64
+  int pad2_;
 };
 
 // Per-wavelength parameters of a phasor launch: the phase opd * inv_wl and the
@@ -116,6 +120,19 @@ __host__ __device__ inline long long phasor_threads(long long rows_pad, int kpad
   return rows_pad * (kpad / 16) * ((nwave + PH_WAVES - 1) / PH_WAVES);
 }
 
+// One packed operand from complex data (mftx_dft2_many): src (or src^T) times
+// the fold cis(fold * (x0 + k)) along K.  `begin` is the op's first thread.
+struct PackOp {
+  const void* src;
+  int dtype, trans;
+  long long m, n, lds;
+  double fold, x0;
+  void* dst;
+  long long rows_pad;
+  int kpad, pad_;
+  long long begin;
+};
+
 // launchers (mftx_gemm.hip, mftx_kernels.hip)
 // bn: output columns per workgroup (64 or 128, see valid_bn).
 hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, const Problem* probs,
@@ -126,6 +143,8 @@ hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, lo
                        hipStream_t s);
 // every op of a plan in one launch (ops_dev on the device, sorted by begin)
 hipError_t launch_phasor_group(int prec, const PhasorOp* ops_dev, int nops, long long total_threads, hipStream_t s);
+// every complex pack op in one launch (ops_dev on the device, sorted by begin)
+hipError_t launch_pack_group(int prec, const PackOp* ops_dev, int nops, long long total_threads, hipStream_t s);
 hipError_t launch_phasor_field(const void* amp, int amp_dtype, long long ld_amp, const void* opd,
                                int opd_dtype, long long ld_opd, int m, int n, double inv_wavelength,
                                void* dst, hipStream_t s);
@@ -141,9 +160,10 @@ hipError_t launch_multiply(const void* a, long long lda, const void* b, long lon
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_weight_field(const void* F, long long ldF, const void* w, int w_dtype, long long ldw,
                                double scale, void* out, long long ldo, int h, int wd, hipStream_t s);
-hipError_t launch_phase_grad(const void* amp, int amp_dtype, long long ld_amp, const void* opd,
-                             int opd_dtype, long long ld_opd, int m, int n, double inv_wl, const void* D,
-                             long long ldD, double* g_opd, long long ldg, double* g_amp, long long ldga,
-                             hipStream_t s);
+// nwave wavelengths (inv_wls on the device, or inv_wl0 alone when null), D_w at D + w * strideD
+hipError_t launch_phase_grad(const void* amp, int amp_dtype, long long ld_amp, const void* opd, int opd_dtype,
+                             long long ld_opd, int m, int n, const double* inv_wls, double inv_wl0, int nwave,
+                             const void* D, long long ldD, long long strideD, double* g_opd, long long ldg,
+                             double* g_amp, long long ldga, hipStream_t s);
 
 }  // namespace mftx

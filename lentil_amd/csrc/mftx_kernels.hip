@@ -83,6 +83,45 @@ hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, lo
   return hipGetLastError();
 }
 
+// Grouped form of pack_kernel: one thread per (op, row, 16-wide K block); the
+// op is found by binary search on its first thread.
+__global__ void pack_group_kernel(int prec, const PackOp* __restrict__ ops, int nops, long long total) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = nops - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ops[mid].begin <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const PackOp op = ops[lo];
+  const long long u = t - op.begin;
+  const int kblocks = op.kpad / 16;
+  const long long row = u / kblocks;
+  const int kb = (int)(u % kblocks);
+  float2 v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const long long k = (long long)kb * 16 + i;
+    const long long r = op.trans ? k : row, c = op.trans ? row : k;
+    if (r < op.m && c < op.n) {
+      const float2 e = load_elem(op.src, op.dtype, r * op.lds + c);
+      v[i] = op.fold == 0.0 ? e : cmul(e, cis_turns(op.fold * (op.x0 + (double)k)));
+    } else {
+      v[i] = make_float2(0.f, 0.f);
+    }
+  }
+  store_packed16(prec, op.dst, packed_lda(prec, op.kpad), row, kb, v);
+}
+
+hipError_t launch_pack_group(int prec, const PackOp* ops, int nops, long long total, hipStream_t s) {
+  if (total <= 0 || nops <= 0) return hipSuccess;
+  const int bs = 256;
+  hipLaunchKernelGGL(pack_group_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, ops, nops,
+                     total);
+  return hipGetLastError();
+}
+
 // --------------------------------------------------------------------------
 // pupil phasor  f = amp * exp(2*pi*i*opd/lambda)   (plane.py:507)
 // --------------------------------------------------------------------------
@@ -274,13 +313,16 @@ hipError_t launch_weight_field(const void* F, long long ldF, const void* w, int 
   return hipGetLastError();
 }
 
-// Given D = dL/d conj(f) for f = amp * exp(2 pi i opd / lambda):
-//   grad_opd += (2 pi / lambda) * (-2) Im(conj(D) f)
-//   grad_amp += 2 Re(conj(D) exp(2 pi i opd / lambda))      (if grad_amp)
-// f is recomputed from amp and opd (opd already includes any diversity).
+// Given D_w = dL/d conj(f_w) for f_w = amp * exp(2 pi i opd / lambda_w),
+// w = 0 .. nwave-1 (D_w at D + w * strideD):
+//   grad_opd += sum_w (2 pi / lambda_w) * (-2) Im(conj(D_w) f_w)
+//   grad_amp += sum_w 2 Re(conj(D_w) exp(2 pi i opd / lambda_w))      (if grad_amp)
+// f is recomputed from amp and opd (opd already includes any diversity); the
+// 1/lambda values are inv_wls[w] on the device, or inv_wl0 when inv_wls is null.
 __global__ void phase_grad_kernel(const void* __restrict__ amp, int amp_dtype, long long ld_amp,
                                   const void* __restrict__ opd, int opd_dtype, long long ld_opd, int m, int n,
-                                  double inv_wl, const float2* __restrict__ D, long long ldD,
+                                  const double* __restrict__ inv_wls, double inv_wl0, int nwave,
+                                  const float2* __restrict__ D, long long ldD, long long strideD,
                                   double* __restrict__ g_opd, long long ldg, double* __restrict__ g_amp,
                                   long long ldga) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,23 +330,31 @@ __global__ void phase_grad_kernel(const void* __restrict__ amp, int amp_dtype, l
   const long long r = t / n, c = t % n;
   const double a = amp_dtype == 3 ? ((const double*)amp)[r * ld_amp + c] : (double)((const float*)amp)[r * ld_amp + c];
   const double o = opd_dtype == 3 ? ((const double*)opd)[r * ld_opd + c] : (double)((const float*)opd)[r * ld_opd + c];
-  const float2 e = cis_turns(o * inv_wl);
-  const float2 d = D[r * ldD + c];
-  // conj(D) * e = (d.x - i d.y)(e.x + i e.y)
-  const double re = (double)d.x * e.x + (double)d.y * e.y;
-  const double im = (double)d.x * e.y - (double)d.y * e.x;
-  g_opd[r * ldg + c] += -4.0 * 3.14159265358979323846 * inv_wl * a * im;
-  if (g_amp) g_amp[r * ldga + c] += 2.0 * re;
+  double go = 0.0, ga = 0.0;
+  for (int w = 0; w < nwave; ++w) {
+    const double inv_wl = inv_wls ? inv_wls[w] : inv_wl0;
+    const float2 e = cis_turns(o * inv_wl);
+    const float2 d = D[(long long)w * strideD + r * ldD + c];
+    // conj(D) * e = (d.x - i d.y)(e.x + i e.y)
+    const double re = (double)d.x * e.x + (double)d.y * e.y;
+    const double im = (double)d.x * e.y - (double)d.y * e.x;
+    go += -4.0 * 3.14159265358979323846 * inv_wl * a * im;
+    ga += 2.0 * re;
+  }
+  g_opd[r * ldg + c] += go;
+  if (g_amp) g_amp[r * ldga + c] += ga;
 }
 
 hipError_t launch_phase_grad(const void* amp, int amp_dtype, long long ld_amp, const void* opd, int opd_dtype,
-                             long long ld_opd, int m, int n, double inv_wl, const void* D, long long ldD,
-                             double* g_opd, long long ldg, double* g_amp, long long ldga, hipStream_t s) {
+                             long long ld_opd, int m, int n, const double* inv_wls, double inv_wl0, int nwave,
+                             const void* D, long long ldD, long long strideD, double* g_opd, long long ldg,
+                             double* g_amp, long long ldga, hipStream_t s) {
   const long long total = (long long)m * n;
-  if (total <= 0) return hipSuccess;
+  if (total <= 0 || nwave <= 0) return hipSuccess;
   const int bs = 256;
   hipLaunchKernelGGL(phase_grad_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, amp, amp_dtype, ld_amp,
-                     opd, opd_dtype, ld_opd, m, n, inv_wl, (const float2*)D, ldD, g_opd, ldg, g_amp, ldga);
+                     opd, opd_dtype, ld_opd, m, n, inv_wls, inv_wl0, nwave, (const float2*)D, ldD, strideD, g_opd,
+                     ldg, g_amp, ldga);
   return hipGetLastError();
 }
 

@@ -4,6 +4,8 @@ Tolerances: the FP32 form is held to rel-L2 <= 2e-6 on complex fields; the
 split-BF16x3 form to rel-L2 <= 2e-5 on complex fields; PSF intensities of
 both forms to rel-L2 <= 1e-5 (the workload gate).
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -130,3 +132,122 @@ def test_large_phase_arguments(L):
     ref = mft.dft2(f, alpha, (96, 80), shift, offset)
     got = L.fourier.dft2(f, alpha, (96, 80), shift, offset)
     assert rel_l2(got, ref) < 5 * TOL_FIELD['fp32']
+
+
+def _many(L, entries, precision):
+    lib, ctx = L._lib.load(), L._lib.context()
+    arr = (L._lib.MftxMft * len(entries))(*entries)
+    L._lib.check(lib.mftx_dft2_many(ctx, arr, len(entries), L._lib.precision_code(precision)), ctx,
+                 'mftx_dft2_many')
+    torch.cuda.synchronize()
+
+
+def _mft_entry(L, F, alpha, shift, offset, scale, conj=0, f=None, amp=None, opd=None, inv_wl=0.0, w=None):
+    q = L._lib.MftxMft()
+    if f is not None:
+        q.f, q.f_dtype, q.m, q.n, q.ldf = f.data_ptr(), L._lib.dtype_code(f), f.shape[0], f.shape[1], f.stride(0)
+    else:
+        q.amp, q.amp_dtype, q.ld_amp = amp.data_ptr(), L._lib.dtype_code(amp), amp.stride(0)
+        q.opd, q.opd_dtype, q.ld_opd = opd.data_ptr(), L._lib.dtype_code(opd), opd.stride(0)
+        q.m, q.n, q.inv_wl = amp.shape[0], amp.shape[1], inv_wl
+    q.F, q.M, q.N, q.ldF = F.data_ptr(), F.shape[0], F.shape[1], F.stride(0)
+    q.alpha_r, q.alpha_c = alpha
+    q.shift_r, q.shift_c = shift
+    q.offset_r, q.offset_c = offset
+    q.scale, q.conj = scale, conj
+    if w is not None:
+        q.w, q.w_dtype, q.ldw = w.data_ptr(), L._lib.dtype_code(w), w.stride(0)
+    return q
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_dft2_many_vs_oracle(L, precision):
+    """One grouped call over a mixed batch: every CASES geometry as complex128
+    input with forward and conjugate twiddles, then phasor inputs of one pupil
+    at four wavelengths with a float32 weight map (the entries of the
+    phase-retrieval backward), each output against the complex128 oracle."""
+    rng = np.random.default_rng(11)
+    entries, refs, keep = [], [], []
+    for i, (m, n, M, N, alpha, shift, offset) in enumerate(CASES):
+        f = _cplx(rng, m, n)
+        conj = i % 2
+        fd = torch.from_numpy(f).cuda()
+        F = torch.empty((M, N), dtype=torch.complex64, device='cuda')
+        unit = float(np.sqrt(abs(alpha[0] * alpha[1])))
+        entries.append(_mft_entry(L, F, alpha, shift, offset, unit, conj, f=fd))
+        if conj:
+            refs.append(np.conj(mft.dft2(np.conj(f), alpha, (M, N), shift, offset, unitary=True)))
+        else:
+            refs.append(mft.dft2(f, alpha, (M, N), shift, offset, unitary=True))
+        keep += [fd, F]
+    n = 64
+    yy, xx = np.mgrid[:n, :n] - n / 2
+    amp = (np.hypot(yy, xx) < n / 2 - 1).astype(float)
+    opd = 3e-7 * (yy / n) ** 2 - 2e-7 * (xx / n) * (yy / n) + 1e-7 * rng.standard_normal((n, n)) * amp
+    ad, od = torch.from_numpy(amp).cuda(), torch.from_numpy(opd).cuda()
+    M = N = 96
+    alpha = (0.0123, 0.0117)
+    w = rng.uniform(0.2, 1.5, (M, N)).astype(np.float32)
+    wd = torch.from_numpy(w).cuda()
+    for wl in (500e-9, 600e-9, 700e-9, 800e-9):
+        F = torch.empty((M, N), dtype=torch.complex64, device='cuda')
+        entries.append(_mft_entry(L, F, alpha, (0.7, -1.3), (0.0, 0.0), 0.37, amp=ad, opd=od, inv_wl=1 / wl, w=wd))
+        f = amp * np.exp(2j * np.pi * opd / wl)
+        refs.append(0.37 * w * mft.dft2(f, alpha, (M, N), (0.7, -1.3), (0, 0), unitary=False))
+        keep.append(F)
+    _many(L, entries, precision)
+    outs = [t for t in keep if t.dtype == torch.complex64]
+    for i, (F, ref) in enumerate(zip(outs, refs)):
+        err = rel_l2(F.cpu().numpy(), ref)
+        print(f'dft2_many {precision} entry {i}: rel-L2 {err:.2e}', flush=True)
+        assert err < TOL_FIELD[precision]
+
+
+def test_dft2_many_split_batch(L):
+    """2000 small MFTs of assorted shapes: more descriptors than one upload
+    slot holds, so the call splits the batch; every output against the oracle."""
+    rng = np.random.default_rng(12)
+    entries, data = [], []
+    for i in range(2000):
+        m, n, M, N = (int(v) for v in rng.integers(3, 13, 4))
+        alpha = tuple(rng.uniform(0.02, 0.2, 2))
+        shift = tuple(rng.uniform(-2, 2, 2))
+        f = _cplx(rng, m, n)
+        fd = torch.from_numpy(f).cuda()
+        F = torch.empty((M, N), dtype=torch.complex64, device='cuda')
+        entries.append(_mft_entry(L, F, alpha, shift, (0.0, 0.0), 1.0, f=fd))
+        data.append((f, alpha, shift, fd, F))
+    _many(L, entries, 'fp32')
+    worst = 0.0
+    for f, alpha, shift, _, F in data:
+        ref = mft.dft2(f, alpha, F.shape, shift, unitary=False)
+        worst = max(worst, rel_l2(F.cpu().numpy(), ref))
+    print(f'dft2_many split batch: worst rel-L2 {worst:.2e}', flush=True)
+    assert worst < TOL_FIELD['fp32']
+
+
+def test_phase_grad_many(L):
+    """The multi-wavelength phase gradient equals the sum of the complex128
+    per-wavelength formula."""
+    rng = np.random.default_rng(13)
+    m, n, nw = 40, 56, 5
+    amp = rng.uniform(0, 1, (m, n))
+    opd = rng.normal(0, 2e-7, (m, n))
+    wls = np.linspace(500e-9, 800e-9, nw)
+    D = (rng.standard_normal((nw, m, n)) + 1j * rng.standard_normal((nw, m, n))).astype(np.complex64)
+    g0 = rng.standard_normal((m, n))
+    ga0 = rng.standard_normal((m, n))
+    lib, ctx = L._lib.load(), L._lib.context()
+    ad, od, Dd = torch.from_numpy(amp).cuda(), torch.from_numpy(opd).cuda(), torch.from_numpy(D).cuda()
+    g, ga = torch.from_numpy(g0.copy()).cuda(), torch.from_numpy(ga0.copy()).cuda()
+    wl_arr = (ctypes.c_double * nw)(*wls)
+    L._lib.check(lib.mftx_phase_grad_many(ctx, ad.data_ptr(), 3, n, od.data_ptr(), 3, n, m, n, wl_arr, nw,
+                                          Dd.data_ptr(), n, m * n, g.data_ptr(), n, ga.data_ptr(), n),
+                 ctx, 'mftx_phase_grad_many')
+    torch.cuda.synchronize()
+    e = np.exp(2j * np.pi * opd[None] / wls[:, None, None])
+    cd = np.conj(D.astype(complex)) * e
+    ref_g = g0 + np.sum(-4 * np.pi / wls[:, None, None] * amp[None] * cd.imag, axis=0)
+    ref_ga = ga0 + np.sum(2 * cd.real, axis=0)
+    assert rel_l2(g.cpu().numpy() - g0, ref_g - g0) < 1e-6
+    assert rel_l2(ga.cpu().numpy() - ga0, ref_ga - ga0) < 1e-6
