@@ -2,6 +2,7 @@
 
 Drop-in surface of lentil v0.8.8 for the propagation hot path
 (lentil/__init__.py:7-79): ``fourier.dft2/idft2``, ``propagate_dft``,
+``propagate_fft``/``scratch_shape``,
 ``Wavefront``, ``Field`` and ``field.insert/merge/reduce``, the planes
 ``Plane/Pupil/Image/Tilt/DispersiveTilt`` and the ptype tokens, plus the
 pupil-building helpers.  The array work runs in libmftx (HIP, gfx950).
@@ -18,7 +19,7 @@ from . import extent, field, fourier, helper, util  # noqa: F401
 from .fourier import set_precision, get_precision  # noqa: F401
 from .field import Field  # noqa: F401
 from .plane import Plane, Pupil, Image, Tilt, DispersiveTilt  # noqa: F401
-from .propagate import propagate_dft  # noqa: F401
+from .propagate import propagate_dft, propagate_fft, scratch_shape  # noqa: F401
 from .wavefront import Wavefront  # noqa: F401
 from .shape import circle, hexagon, rectangle, spider  # noqa: F401
 from .segmented import hex_segments  # noqa: F401

@@ -57,10 +57,14 @@ class Wavefront:
             for f in self.data:
                 out = _field.insert(f, out)
             return out
-        acc = torch.zeros(tuple(self.shape), dtype=torch.complex128, device=_lib.device())
+        return self.field_tensor().cpu().numpy()
+
+    def field_tensor(self, dtype=torch.complex128):
+        """Device-resident complex field on the wavefront grid (no host copy)."""
+        acc = torch.zeros(tuple(self.shape), dtype=dtype, device=_lib.device())
         for f in self.data:
             _field.insert(f, acc)
-        return acc.cpu().numpy()
+        return acc
 
     @property
     def intensity(self):

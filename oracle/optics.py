@@ -486,6 +486,85 @@ def propagate_dft(w, pixelscale, shape=None, prop_shape=None, oversample=2, mask
     return out
 
 
+# --------------------------------------------------------------------------
+# FFT propagation (propagate.py:9-135)
+# --------------------------------------------------------------------------
+
+
+def pad(array, shape):
+    """util.py:31-90 for 2-D arrays: centred zero pad, or a centred crop along
+    an axis where the array is the larger."""
+    array = np.asarray(array)
+    out = np.zeros((shape[0], shape[1]), dtype=array.dtype)
+    src, dst = [], []
+    for a, b in zip(array.shape, shape):
+        if b - a <= 0:
+            lo = (a - b) // 2
+            src.append(slice(lo, lo + b))
+            dst.append(slice(0, b))
+        else:
+            lo = (b - a) // 2
+            src.append(slice(0, a))
+            dst.append(slice(lo, lo + a))
+    out[tuple(dst)] = array[tuple(src)]
+    return out
+
+
+def fft_shape(dx, du, z, wavelength, oversample):
+    """propagate.py:126-131: the FFT grid is round(1/alpha) per axis and the
+    propagation wavelength is recomputed for that integer grid."""
+    alpha = np.asarray(dft_alpha(dx, du, wavelength, z, oversample))
+    n = np.round(np.reciprocal(alpha)).astype(int)
+    return n, np.min((n / oversample * dx * du) / z)
+
+
+def scratch_shape(wavelength, dx, du, z, oversample):
+    """propagate.py:87-118."""
+    n, _ = fft_shape(np.broadcast_to(dx, (2,)), np.broadcast_to(du, (2,)), z, np.max(wavelength), oversample)
+    return tuple(n)
+
+
+def fft2_centred(x):
+    """propagate.py:134-135: ifftshift(fft2(fftshift(x)), norm='ortho')."""
+    return np.fft.ifftshift(np.fft.fft2(np.fft.fftshift(x), norm='ortho'))
+
+
+def propagate_fft(w, pixelscale, shape=None, oversample=2, scratch=None):
+    """propagate.py:9-84 (complex128).  Returns the propagated Wavefront; with
+    ``scratch`` the padded input is left in scratch[:N0, :N1] as in the
+    reference."""
+    if any(f.tilt for f in w.data):
+        raise NotImplementedError('propagate_fft does not support Wavefronts with fitted tilt. '
+                                  'Use propagate_dft instead.')
+    if w.ptype not in ('pupil', 'image'):
+        raise TypeError("Wavefront must have ptype 'pupil' or 'image'")
+    ptype_out = 'image' if w.ptype == 'pupil' else 'pupil'
+    pixelscale = np.broadcast_to(pixelscale, (2,))
+    n, prop_wl = fft_shape(w.pixelscale, pixelscale, w.focal_length, w.wavelength, oversample)
+    if shape is None:
+        shape_out = tuple(int(v) for v in n)
+    else:
+        shape = tuple(np.broadcast_to(shape, (2,)))
+        if np.any(shape > n / oversample):
+            raise ValueError(f'requested shape {tuple(shape)} is larger in at least one dimension than '
+                             f'maximum propagation shape {tuple(n // oversample)}')
+        shape_out = (shape[0] * oversample, shape[1] * oversample)
+    out = Wavefront.empty(prop_wl, pixelscale=pixelscale / oversample, focal_length=w.focal_length,
+                          shape=shape_out, ptype=ptype_out)
+    if scratch is not None:
+        if not all(np.asarray(scratch.shape) > n):
+            raise ValueError(f'scratch must have shape greater than or equal to {tuple(n)}')
+        view = scratch[0:n[0], 0:n[1]]
+        view[...] = 0
+        for f in w.data:
+            view[...] = insert(f, view)
+        data = fft2_centred(view)
+    else:
+        data = fft2_centred(pad(w.field, n))
+    out.data.append(Field(data=data, pixelscale=pixelscale / oversample))
+    return out
+
+
 def broadband_image(pupil, wavelengths, weights, pixelscale, shape, oversample=2,
                     prop_shape=None, mask=None, tilts=()):
     """The reference's user loop (docs/user/diffraction.rst:148-160):
