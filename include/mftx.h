@@ -164,6 +164,38 @@ typedef struct {
 } mftx_mft;
 int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision);
 
+/* ---------------------------------------------------------------------------
+ * Detector-grid post-processing of an accumulated image (device pointers,
+ * real float32 (1) / float64 (3) data, row strides in elements).
+ * ------------------------------------------------------------------------- */
+
+/* Integer block sums of nimg stacked rows x cols images (image i at
+ * img + i * rows * ld) into nimg contiguous (rows/factor) x (cols/factor)
+ * outputs: util.rebin (util.py:221-259). */
+int mftx_rebin(mftx_ctx* ctx, const void* img, int dtype, int64_t nimg, int64_t rows, int64_t cols, int64_t ld,
+               int64_t factor, void* out, int out_dtype);
+
+/* out = |ifft2(fft2(img) * kernel)|, times sum(img) / sum(out) when
+ * normalize: the MTF convolution of detector.pixel (detector.py:216-225),
+ * convolvable.jitter (convolvable.py:72-82) and convolvable.smear
+ * (convolvable.py:152-169); kernel is rows x cols in numpy's fftfreq order.
+ * Both transforms are MFTs with alpha = 1/N; the kernel multiplies the
+ * spectrum in the forward transform's store. */
+int mftx_fft_filter(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_t cols, int64_t ld,
+                    const void* kernel, int kernel_dtype, int64_t ldk, int normalize, void* out, int out_dtype,
+                    int64_t ldo, int precision);
+
+/* util.rescale (util.py:262-349) with mode 'nearest': resample img to
+ * out_rows x out_cols at the coordinates (i - out/2) / scale + in/2 with
+ * spline order 0, 1 or 3 (scipy map_coordinates semantics: 12 samples of edge
+ * padding and the cubic B-spline prefilter for order 3); the mask (img != 0
+ * when mask is NULL) is resampled with order 1 and zeroed below its dtype's
+ * eps; unitary scales by sum(img) / sum(resampled); the output is the product
+ * with the mask. */
+int mftx_rescale(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_t cols, int64_t ld, const void* mask,
+                 int mask_dtype, int64_t ldm, double scale, int64_t out_rows, int64_t out_cols, int order, int unitary,
+                 void* out, int out_dtype, int64_t ldo);
+
 /* Fill `bytes` bytes at device pointer p with zeros (a kernel, not a memset node). */
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes);
 

@@ -75,6 +75,11 @@ SIGNATURES = {
     'mftx_phase_grad_many': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, c_i64, c_vp,
                                      ctypes.c_int32, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64]),
     'mftx_zero': (c_int, [c_vp, c_vp, c_i64]),
+    'mftx_rebin': (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_int]),
+    'mftx_fft_filter': (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_vp, c_int, c_i64, c_int, c_vp, c_int,
+                                c_i64, c_int]),
+    'mftx_rescale': (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_vp, c_int, c_i64, c_dbl, c_i64, c_i64,
+                             c_int, c_int, c_vp, c_int, c_i64]),
     'mftx_weight_field': (c_int, [c_vp, c_vp, c_i64, c_vp, c_int, c_i64, c_dbl, c_vp, c_i64, c_i64, c_i64]),
     'mftx_phase_grad': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, c_i64, c_dbl, c_vp, c_i64,
                                 c_vp, c_i64, c_vp, c_i64]),

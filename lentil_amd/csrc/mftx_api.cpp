@@ -117,6 +117,8 @@ struct mftx_ctx {
   std::string err;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  void* aux = nullptr;  // intermediates of the detector operators (ws holds their MFT operands)
+  size_t aux_bytes = 0;
   // descriptor ring: pinned host staging + device slots, reused round-robin
   char* ring_host = nullptr;
   char* ring_dev = nullptr;
@@ -223,6 +225,22 @@ static int commit_desc(mftx_ctx* ctx, int s, size_t bytes) {
   HIPCHK(ctx, hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ring_ev[s], ctx->stream));
   ctx->ring_used[s] = true;
+  return MFTX_OK;
+}
+
+static int ensure_aux(mftx_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->aux_bytes) return MFTX_OK;
+  if (ctx->aux) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipFree(ctx->aux));
+    ctx->aux = nullptr;
+    ctx->aux_bytes = 0;
+  }
+  if (hipMalloc(&ctx->aux, bytes) != hipSuccess) {
+    ctx->aux = nullptr;
+    return fail(ctx, MFTX_ENOMEM, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  ctx->aux_bytes = bytes;
   return MFTX_OK;
 }
 
@@ -528,6 +546,7 @@ int mftx_destroy(mftx_ctx* ctx) {
   if (ctx->ring_host) (void)hipHostFree(ctx->ring_host);
   if (ctx->ring_dev) (void)hipFree(ctx->ring_dev);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->aux) (void)hipFree(ctx->aux);
   delete ctx;
   return MFTX_OK;
 }
@@ -753,6 +772,129 @@ int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t p
   if (count == 0) return MFTX_OK;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   return dft2_many_run(ctx, mfts, count, precision);
+}
+
+// ---------------------------------------------------------------------------
+// detector-grid post-processing
+// ---------------------------------------------------------------------------
+
+int mftx_rebin(mftx_ctx* ctx, const void* img, int dtype, int64_t nimg, int64_t rows, int64_t cols, int64_t ld,
+               int64_t factor, void* out, int out_dtype) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!img || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if ((dtype != 1 && dtype != 3) || (out_dtype != 1 && out_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "rebin is defined for float32/float64 data");
+  if (nimg <= 0 || rows <= 0 || cols <= 0 || ld < cols || factor <= 0 || factor > (1 << 16))
+    return fail(ctx, MFTX_EINVAL, "bad rebin shape");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, launch_rebin(img, dtype, nimg, rows, cols, ld, (int)factor, out, out_dtype, ctx->stream));
+  return MFTX_OK;
+}
+
+// wrap v modulo n into (-n/2, n/2]
+static double wrap_coord(int64_t v, int64_t n) {
+  int64_t w = ((v % n) + n) % n;
+  if (2 * w > n) w -= n;
+  return (double)w;
+}
+
+int mftx_fft_filter(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_t cols, int64_t ld,
+                    const void* kernel, int kernel_dtype, int64_t ldk, int normalize, void* out, int out_dtype,
+                    int64_t ldo, int precision) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!img || !kernel || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if ((dtype != 1 && dtype != 3) || (kernel_dtype != 1 && kernel_dtype != 3) || (out_dtype != 1 && out_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "image, kernel and output must be float32 or float64");
+  if (rows <= 0 || cols <= 0 || ld < cols || ldk < cols || ldo < cols || rows > (1 << 20) || cols > (1 << 20))
+    return fail(ctx, MFTX_EINVAL, "bad filter shape");
+  if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3) return fail(ctx, MFTX_EINVAL, "unknown precision");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int64_t n = rows * cols;
+  const int64_t nb = (n + 255) / 256;
+  const size_t fbytes = (size_t)round_up(n * (int64_t)sizeof(float2), 256);
+  int rc = ensure_aux(ctx, 2 * fbytes + sizeof(double) * (size_t)(2 * nb + 2));
+  if (rc) return rc;
+  char* a = (char*)ctx->aux;
+  float2* F = (float2*)a;
+  float2* Y = (float2*)(a + fbytes);
+  double* part = (double*)(a + 2 * fbytes);
+  double* sums = part + 2 * nb;
+  // fft2 then ifft2 as MFTs with alpha = 1/N: input coordinate r and output
+  // coordinate u (uncentred) are r - N//2 + N//2 and (u - N//2) + N//2,
+  // offsets taken modulo N
+  mftx_mft e = {};
+  e.f = img;
+  e.f_dtype = dtype;
+  e.m = rows;
+  e.n = cols;
+  e.ldf = ld;
+  e.F = F;
+  e.M = rows;
+  e.N = cols;
+  e.ldF = cols;
+  e.alpha_r = 1.0 / (double)rows;
+  e.alpha_c = 1.0 / (double)cols;
+  e.offset_r = wrap_coord(rows / 2, rows);
+  e.offset_c = wrap_coord(cols / 2, cols);
+  e.shift_r = wrap_coord(-(rows / 2), rows);
+  e.shift_c = wrap_coord(-(cols / 2), cols);
+  e.scale = 1.0;
+  e.w = kernel;  // the MTF multiplies the spectrum in the store
+  e.w_dtype = kernel_dtype;
+  e.ldw = ldk;
+  rc = dft2_many_run(ctx, &e, 1, precision);
+  if (rc) return rc;
+  mftx_mft b = e;
+  b.f = F;
+  b.f_dtype = 0;
+  b.ldf = cols;
+  b.F = Y;
+  b.conj = 1;
+  b.scale = 1.0 / ((double)rows * (double)cols);
+  b.w = nullptr;
+  rc = dft2_many_run(ctx, &b, 1, precision);
+  if (rc) return rc;
+  HIPCHK(ctx, launch_magnitude(Y, cols, rows, cols, out, out_dtype, ldo, part, sums + 1, ctx->stream));
+  if (normalize) {
+    HIPCHK(ctx, launch_sum(img, dtype, rows, cols, ld, part + nb, sums, ctx->stream));
+    HIPCHK(ctx, launch_scale_ratio(out, out_dtype, rows, cols, ldo, sums, sums + 1, ctx->stream));
+  }
+  return MFTX_OK;
+}
+
+int mftx_rescale(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_t cols, int64_t ld, const void* mask,
+                 int mask_dtype, int64_t ldm, double scale, int64_t out_rows, int64_t out_cols, int order, int unitary,
+                 void* out, int out_dtype, int64_t ldo) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!img || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if ((dtype != 1 && dtype != 3) || (out_dtype != 1 && out_dtype != 3) || (mask && mask_dtype != 1 && mask_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "image, mask and output must be float32 or float64");
+  if (order != 0 && order != 1 && order != 3) return fail(ctx, MFTX_EINVAL, "spline order must be 0, 1 or 3");
+  if (rows <= 0 || cols <= 0 || ld < cols || out_rows <= 0 || out_cols <= 0 || ldo < out_cols ||
+      (mask && ldm < cols) || rows > (1 << 20) || cols > (1 << 20) || out_rows > (1 << 20) || out_cols > (1 << 20))
+    return fail(ctx, MFTX_EINVAL, "bad rescale shape");
+  if (!(scale > 0) || !std::isfinite(scale)) return fail(ctx, MFTX_EINVAL, "scale must be positive");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int npad = order > 1 ? 12 : 0;
+  const int64_t np_ = (rows + 2 * npad) * (cols + 2 * npad), no = out_rows * out_cols;
+  const int64_t nb = (std::max(rows * cols, no) + 255) / 256;
+  const size_t b_pad = (size_t)round_up(np_ * 8, 256), b_out = (size_t)round_up(no * 8, 256);
+  int rc = ensure_aux(ctx, 2 * b_pad + 2 * b_out + sizeof(double) * (size_t)(nb + 2));
+  if (rc) return rc;
+  char* a = (char*)ctx->aux;
+  double* P = (double*)a;
+  double* T = (double*)(a + b_pad);
+  double* raw = (double*)(a + 2 * b_pad);
+  double* mval = (double*)(a + 2 * b_pad + b_out);
+  double* part = (double*)(a + 2 * b_pad + 2 * b_out);
+  double* sums = part + nb;
+  // the mask's dtype sets the threshold (util.py:339, numpy finfo eps)
+  const int mdt = mask ? mask_dtype : dtype;
+  const double eps = mdt == 3 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+  HIPCHK(ctx, launch_rescale(img, dtype, rows, cols, ld, mask ? mask : img, mask ? 2 : 1, mdt, mask ? ldm : ld, eps,
+                             scale, out_rows, out_cols, order, unitary, out, out_dtype, ldo, P, T, raw, mval, part,
+                             sums, ctx->stream));
+  return MFTX_OK;
 }
 
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes) {

@@ -166,4 +166,25 @@ hipError_t launch_phase_grad(const void* amp, int amp_dtype, long long ld_amp, c
                              const void* D, long long ldD, long long strideD, double* g_opd, long long ldg,
                              double* g_amp, long long ldga, hipStream_t s);
 
+// detector-grid post-processing (mftx_detector.hip)
+hipError_t launch_rebin(const void* src, int dtype, long long nimg, long long rows, long long cols, long long ld,
+                        int factor, void* dst, int odt, hipStream_t s);
+// *out = sum of a real array (part: one double per 256 elements)
+hipError_t launch_sum(const void* x, int dtype, long long rows, long long cols, long long ld, double* part,
+                      double* out, hipStream_t s);
+// out = |F| (F complex64), *sum = sum of out (part as for launch_sum)
+hipError_t launch_magnitude(const void* F, long long ldF, long long rows, long long cols, void* out, int odt,
+                            long long ldo, double* part, double* sum, hipStream_t s);
+// x *= *num / *den
+hipError_t launch_scale_ratio(void* x, int dtype, long long rows, long long cols, long long ld, const double* num,
+                              const double* den, hipStream_t s);
+// map_coordinates(order 0/1/3, mode 'nearest') resampling with the mask and
+// unitary scaling of rescale; P, T: (rows + 24) x (cols + 24) doubles; raw,
+// mval: orows x ocols doubles; part: one double per 256 elements of the
+// larger of the input and output; sums: 2 doubles
+hipError_t launch_rescale(const void* img, int dtype, long long rows, long long cols, long long ld, const void* msrc,
+                          int mask_mode, int mdt, long long ldm, double eps, double scale, long long orows,
+                          long long ocols, int order, int unitary, void* out, int odt, long long ldo, double* P,
+                          double* T, double* raw, double* mval, double* part, double* sums, hipStream_t s);
+
 }  // namespace mftx

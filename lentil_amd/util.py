@@ -63,3 +63,94 @@ def sanitize_bandpass(vec):
     """util.py:424-428."""
     vec = np.asarray(vec)
     return vec[np.newaxis, ...] if vec.shape == () else vec
+
+
+def rebin(img, factor):
+    """Rebin an image (or a cube of images) by an integer factor (util.py:221-259),
+    as block sums on the device.  NumPy in gives NumPy out with the reference's
+    dtype; a float32/float64 device tensor gives a device tensor."""
+    import ctypes
+    import torch
+    from . import _lib
+    from ._arrays import is_dev, to_dev
+    host = not is_dev(img)
+    if host:
+        a = np.asarray(img)
+        if np.iscomplexobj(a):
+            raise ValueError('rebin is not defined for complex data')
+        ref_dtype = a.dtype if a.ndim == 3 else np.zeros(1, a.dtype).sum().dtype
+    elif img.is_complex():
+        raise ValueError('rebin is not defined for complex data')
+    t = to_dev(img)
+    if t.dtype not in (torch.float32, torch.float64):
+        t = t.to(torch.float64)
+    t = t.contiguous()
+    if t.dim() not in (2, 3):
+        raise ValueError(f'expected an image or a cube of images, got shape {tuple(t.shape)}')
+    factor = int(factor)
+    rows, cols = t.shape[-2:]
+    if factor <= 0 or rows % factor or cols % factor:
+        raise ValueError(f'cannot rebin shape {tuple(t.shape)} by {factor}')
+    nimg = t.shape[0] if t.dim() == 3 else 1
+    out = torch.empty(tuple(t.shape[:-2]) + (rows // factor, cols // factor), dtype=t.dtype, device=t.device)
+    lib = _lib.load()
+    ctx = _lib.context()
+    _lib.check(lib.mftx_rebin(ctx, ctypes.c_void_p(t.data_ptr()), _lib.dtype_code(t), nimg, rows, cols, cols, factor,
+                              ctypes.c_void_p(out.data_ptr()), _lib.dtype_code(out)), ctx, 'mftx_rebin')
+    if host:
+        return out.cpu().numpy().astype(ref_dtype)
+    return out
+
+
+def rescale(img, scale, shape=None, mask=None, order=3, mode='nearest', unitary=True):
+    """Rescale an image by spline interpolation (util.py:262-349) on the device.
+
+    Follows scipy.ndimage.map_coordinates as the reference calls it: the
+    sample coordinates (i - out/2) / scale + in/2, the mask (nonzero pixels
+    when None) resampled with order 1 and zeroed below its dtype's eps, the
+    unitary weight correction, then the product with the mask.  Built for
+    mode 'nearest' (the default, and what pixelate uses) with orders 0, 1
+    and 3, on real images; other modes, orders and complex images raise
+    NotImplementedError.  NumPy in gives NumPy out of the image's float
+    dtype; a device tensor gives a device tensor.
+    """
+    import ctypes
+    import torch
+    from . import _lib
+    from ._arrays import is_dev, to_dev
+    if mode != 'nearest' or order not in (0, 1, 3):
+        raise NotImplementedError("the device rescale supports mode='nearest' with order 0, 1 or 3")
+    host = not is_dev(img)
+    if (np.iscomplexobj(np.asarray(img)) if host else img.is_complex()):
+        raise NotImplementedError('the device rescale supports real images')
+    t = to_dev(img)
+    if t.dtype not in (torch.float32, torch.float64):
+        t = t.to(torch.float64)
+    t = t.contiguous()
+    if t.dim() != 2:
+        raise ValueError(f'expected a 2-D image, got shape {tuple(t.shape)}')
+    rows, cols = t.shape
+    if shape is None:
+        oshape = np.ceil((rows * scale, cols * scale)).astype(int)
+    elif np.isscalar(shape):
+        oshape = np.ceil((shape * scale, shape * scale)).astype(int)
+    else:
+        oshape = np.ceil((shape[0] * scale, shape[1] * scale)).astype(int)
+    m = None
+    if mask is not None:
+        m = to_dev(mask)
+        if m.dtype not in (torch.float32, torch.float64):
+            m = m.to(torch.float64)
+        m = m.contiguous()
+        if tuple(m.shape) != (rows, cols):
+            raise ValueError(f'mask shape {tuple(m.shape)} != image shape {(rows, cols)}')
+    out = torch.empty((int(oshape[0]), int(oshape[1])), dtype=t.dtype, device=t.device)
+    lib = _lib.load()
+    ctx = _lib.context()
+    _lib.check(lib.mftx_rescale(ctx, ctypes.c_void_p(t.data_ptr()), _lib.dtype_code(t), rows, cols, cols,
+                                ctypes.c_void_p(m.data_ptr()) if m is not None else None,
+                                _lib.dtype_code(m) if m is not None else 0, cols, float(scale),
+                                out.shape[0], out.shape[1], int(order), 1 if unitary else 0,
+                                ctypes.c_void_p(out.data_ptr()), _lib.dtype_code(out), out.shape[1]),
+               ctx, 'mftx_rescale')
+    return out.cpu().numpy() if host else out
