@@ -196,6 +196,34 @@ int mftx_rescale(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_
                  int mask_dtype, int64_t ldm, double scale, int64_t out_rows, int64_t out_cols, int order, int unitary,
                  void* out, int out_dtype, int64_t ldo);
 
+/* ---------------------------------------------------------------------------
+ * Modal OPD operations (device pointers; masks: nseg arrays mask_stride
+ * elements apart, float32 (1) / float64 (3)).
+ * ------------------------------------------------------------------------- */
+
+/* Normal equations of Plane.fit_tilt's least-squares piston/tip/tilt fit
+ * (plane.py:521-611) per segment, over the columns
+ * mask * (1, r * dx_r, -c * dx_c) of Plane.ptt_vector with the centred mesh
+ * r, c (helper.py:7-17).  normal (host memory, nseg x 9) receives
+ * G00 G01 G02 G11 G12 G22 b0 b1 b2 per segment; the call synchronises. */
+int mftx_ptt_fit(mftx_ctx* ctx, const void* opd, int opd_dtype, int64_t ld_opd, const void* masks, int mask_dtype,
+                 int64_t ld_mask, int64_t mask_stride, int32_t nseg, int64_t rows, int64_t cols, double dx_r,
+                 double dx_c, double* normal);
+
+/* Remove fitted tilt (plane.py:590-609): out = opd - m (r dx_r t1 - c dx_c t2)
+ * for one segment (multi = 0), or sum_s (opd - m_s (...)) m_s for several;
+ * tilt (host memory) holds (t1, t2) per segment.  out is float64. */
+int mftx_remove_tilt(mftx_ctx* ctx, const void* opd, int opd_dtype, int64_t ld_opd, const void* masks, int mask_dtype,
+                     int64_t ld_mask, int64_t mask_stride, int32_t nseg, int32_t multi, int64_t rows, int64_t cols,
+                     double dx_r, double dx_c, const double* tilt, double* out, int64_t ldo);
+
+/* Zernike basis (zernike.py:11-109,183-234): out[mode][pixel] for nmodes
+ * (n, m) pairs in noll_nm (host memory), at float64 polar coordinates
+ * rho/theta over npix pixels, times the mask's nonzero pixels; Noll's
+ * normalisation when normalize. */
+int mftx_zernike_basis(mftx_ctx* ctx, const double* rho, const double* theta, const void* mask, int mask_dtype,
+                       int64_t npix, const int32_t* noll_nm, int32_t nmodes, int normalize, double* out);
+
 /* Fill `bytes` bytes at device pointer p with zeros (a kernel, not a memset node). */
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes);
 

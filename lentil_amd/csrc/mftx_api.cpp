@@ -897,6 +897,109 @@ int mftx_rescale(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_
   return MFTX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// modal OPD operations
+// ---------------------------------------------------------------------------
+
+int mftx_ptt_fit(mftx_ctx* ctx, const void* opd, int opd_dtype, int64_t ld_opd, const void* masks, int mask_dtype,
+                 int64_t ld_mask, int64_t mask_stride, int32_t nseg, int64_t rows, int64_t cols, double dx_r,
+                 double dx_c, double* normal) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!opd || !masks || !normal) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if ((opd_dtype != 1 && opd_dtype != 3) || (mask_dtype != 1 && mask_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "opd and masks must be float32 or float64");
+  if (rows <= 0 || cols <= 0 || ld_opd < cols || ld_mask < cols || nseg <= 0 || nseg > 65535 ||
+      (nseg > 1 && mask_stride < rows * ld_mask))
+    return fail(ctx, MFTX_EINVAL, "bad tilt-fit shape");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int64_t nblk = (rows * cols + 255) / 256;
+  int rc = ensure_aux(ctx, sizeof(double) * (size_t)(nseg * nblk * 9 + nseg * 9));
+  if (rc) return rc;
+  double* part = (double*)ctx->aux;
+  double* out = part + nseg * nblk * 9;
+  HIPCHK(ctx, launch_ptt_normal(opd, opd_dtype, ld_opd, masks, mask_dtype, ld_mask, mask_stride, nseg, rows, cols,
+                                dx_r, dx_c, part, out, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(normal, out, sizeof(double) * 9 * nseg, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MFTX_OK;
+}
+
+int mftx_remove_tilt(mftx_ctx* ctx, const void* opd, int opd_dtype, int64_t ld_opd, const void* masks, int mask_dtype,
+                     int64_t ld_mask, int64_t mask_stride, int32_t nseg, int32_t multi, int64_t rows, int64_t cols,
+                     double dx_r, double dx_c, const double* tilt, double* out, int64_t ldo) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!opd || !masks || !tilt || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if ((opd_dtype != 1 && opd_dtype != 3) || (mask_dtype != 1 && mask_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "opd and masks must be float32 or float64");
+  if (rows <= 0 || cols <= 0 || ld_opd < cols || ld_mask < cols || ldo < cols || nseg <= 0 ||
+      (!multi && nseg != 1) || (size_t)nseg * 2 * sizeof(double) > kSlotBytes ||
+      (nseg > 1 && mask_stride < rows * ld_mask))
+    return fail(ctx, MFTX_EINVAL, "bad tilt-removal shape");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  void* dev = nullptr;
+  const int rc = upload_desc(ctx, tilt, sizeof(double) * 2 * nseg, &dev);
+  if (rc) return rc;
+  HIPCHK(ctx, launch_remove_tilt(opd, opd_dtype, ld_opd, masks, mask_dtype, ld_mask, mask_stride, nseg, multi, rows,
+                                 cols, dx_r, dx_c, (const double*)dev, out, ldo, ctx->stream));
+  return MFTX_OK;
+}
+
+int mftx_zernike_basis(mftx_ctx* ctx, const double* rho, const double* theta, const void* mask, int mask_dtype,
+                       int64_t npix, const int32_t* noll_nm, int32_t nmodes, int normalize, double* out) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!rho || !theta || !mask || !noll_nm || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
+  if (mask_dtype != 1 && mask_dtype != 3) return fail(ctx, MFTX_EINVAL, "mask must be float32 or float64");
+  if (npix <= 0 || nmodes <= 0 || nmodes > 65535) return fail(ctx, MFTX_EINVAL, "bad basis shape");
+  // radial coefficients (-1)^k (n-k)! / (k! ((n+m)/2-k)! ((n-m)/2-k)!) (zernike.py:104-107),
+  // integers, exact in double while below 2^53
+  std::vector<int> desc, pw;
+  std::vector<double> coef, norm;
+  for (int32_t i = 0; i < nmodes; ++i) {
+    const int n = noll_nm[2 * i], m = noll_nm[2 * i + 1];
+    const int am = std::abs(m);
+    if (n < 0 || am > n || n > 60) return fail(ctx, MFTX_EINVAL, "bad Zernike (n, m)");
+    desc.push_back(n);
+    desc.push_back(m);
+    desc.push_back((int)coef.size());
+    int nk = 0;
+    if ((n - am) % 2 == 0) {
+      for (int k = 0; k <= (n - am) / 2; ++k) {
+        // exact rational in long double, then rounded once (an integer)
+        long double c = 1.0L;
+        for (int q = 2; q <= n - k; ++q) c *= q;
+        for (int q = 2; q <= k; ++q) c /= q;
+        for (int q = 2; q <= (n + am) / 2 - k; ++q) c /= q;
+        for (int q = 2; q <= (n - am) / 2 - k; ++q) c /= q;
+        coef.push_back((double)((k % 2 ? -1.0L : 1.0L) * std::nearbyint((double)c)));
+        pw.push_back(n - 2 * k);
+        ++nk;
+      }
+    }
+    desc.push_back(nk);
+    norm.push_back(normalize ? (m == 0 ? std::sqrt((double)(n + 1)) : std::sqrt(2.0) * std::sqrt((double)(n + 1)))
+                             : 1.0);
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t o_desc = 0, o_norm = round_up(sizeof(int) * desc.size(), 16);
+  const size_t o_coef = round_up(o_norm + sizeof(double) * norm.size(), 16);
+  const size_t o_pw = round_up(o_coef + sizeof(double) * std::max<size_t>(coef.size(), 1), 16);
+  const size_t bytes = o_pw + sizeof(int) * std::max<size_t>(pw.size(), 1);
+  if (bytes > kSlotBytes) return fail(ctx, MFTX_EINVAL, "too many Zernike modes for one call");
+  std::vector<char> host(bytes);
+  std::memcpy(host.data() + o_desc, desc.data(), sizeof(int) * desc.size());
+  std::memcpy(host.data() + o_norm, norm.data(), sizeof(double) * norm.size());
+  if (!coef.empty()) std::memcpy(host.data() + o_coef, coef.data(), sizeof(double) * coef.size());
+  if (!pw.empty()) std::memcpy(host.data() + o_pw, pw.data(), sizeof(int) * pw.size());
+  void* dev = nullptr;
+  const int rc = upload_desc(ctx, host.data(), bytes, &dev);
+  if (rc) return rc;
+  char* d = (char*)dev;
+  HIPCHK(ctx, launch_zernike(rho, theta, mask, mask_dtype, npix, nmodes, (const int*)(d + o_desc),
+                             (const double*)(d + o_coef), (const int*)(d + o_pw), (const double*)(d + o_norm), out,
+                             ctx->stream));
+  return MFTX_OK;
+}
+
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes) {
   if (!ctx) return MFTX_EHANDLE;
   if (!p || bytes < 0) return fail(ctx, MFTX_EINVAL, "bad zero request");

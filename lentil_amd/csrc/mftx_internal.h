@@ -187,4 +187,17 @@ hipError_t launch_rescale(const void* img, int dtype, long long rows, long long 
                           long long ocols, int order, int unitary, void* out, int odt, long long ldo, double* P,
                           double* T, double* raw, double* mval, double* part, double* sums, hipStream_t s);
 
+// modal OPD operations (mftx_modal.hip)
+// out[seg][9] = normal equations of the piston/tip/tilt fit; part: nseg * ceil(rows*cols/256) * 9 doubles
+hipError_t launch_ptt_normal(const void* opd, int odt, long long ld_opd, const void* masks, int mdt, long long ld_mask,
+                             long long mask_stride, int nseg, long long rows, long long cols, double dx_r, double dx_c,
+                             double* part, double* out, hipStream_t s);
+hipError_t launch_remove_tilt(const void* opd, int odt, long long ld_opd, const void* masks, int mdt,
+                              long long ld_mask, long long mask_stride, int nseg, int multi, long long rows,
+                              long long cols, double dx_r, double dx_c, const double* tilt, double* out,
+                              long long ldo, hipStream_t s);
+hipError_t launch_zernike(const double* rho, const double* theta, const void* mask, int mdt, long long npix,
+                          int nmodes, const int* desc, const double* coef, const int* pw, const double* norm,
+                          double* out, hipStream_t s);
+
 }  // namespace mftx

@@ -217,6 +217,69 @@ class Plane(_PlaneBase):
             self._cache = {'slices': _plane_slice(self.mask), 'tiles': out}
         return out
 
+    @property
+    def ptt_vector(self):
+        """Piston, x-tilt and y-tilt vectors over the raveled plane, three rows
+        per segment, each times the segment mask (plane.py:521-561); None for a
+        plane without shape."""
+        if self.shape == () or self.shape is None:
+            return None
+        if self.pixelscale is None:
+            raise ValueError("can't create ptt_vector with pixelscale = ()")
+        pixelscale = np.broadcast_to(self.pixelscale, (2,))
+        r, c = helper.mesh(self.shape)
+        unmasked = np.einsum('ij,i->ij', [np.ones(r.size), r.ravel(), -c.ravel()], [1, pixelscale[0], pixelscale[1]])
+        if self.size == 1:
+            return np.einsum('ij,j->ij', unmasked, self.mask.ravel())
+        out = np.empty((self.size * 3, np.prod(self.shape)))
+        for k in range(self.size):
+            out[3 * k:3 * k + 3] = unmasked * self.mask[k].ravel()
+        return out
+
+    def fit_tilt(self, inplace=False):
+        """Fit and remove tilt from the OPD by least squares, bookkeeping the
+        angular tilt in ``tilt`` (plane.py:563-611).
+
+        The normal equations of every segment's piston/tip/tilt fit come from
+        one device pass over the OPD and masks (``mftx_ptt_fit``); each 3x3
+        system is solved on the host with the minimum-norm least-squares
+        solution, and the tilt is removed on the device
+        (``mftx_remove_tilt``)."""
+        plane = self if inplace else copy.deepcopy(self)
+        if plane.shape == () or plane.shape is None:
+            return plane
+        if plane.pixelscale is None:
+            raise ValueError("can't create ptt_vector with pixelscale = ()")
+        opd = np.asarray(plane.opd)
+        if opd.size == 1:
+            return plane
+        dx = np.broadcast_to(plane.pixelscale, (2,)).astype(float)
+        rows, cols = plane.shape
+        nseg = plane.size
+        opd_d = to_dev(np.ascontiguousarray(opd, dtype=np.float64))
+        mask_d = to_dev(np.ascontiguousarray(np.asarray(plane.mask), dtype=np.float64))
+        lib = _lib.load()
+        ctx = _lib.context()
+        normal = np.zeros((nseg, 9))
+        _lib.check(lib.mftx_ptt_fit(ctx, ctypes.c_void_p(opd_d.data_ptr()), _lib.DT_F64, cols,
+                                    ctypes.c_void_p(mask_d.data_ptr()), _lib.DT_F64, cols, rows * cols, nseg, rows,
+                                    cols, float(dx[0]), float(dx[1]), normal.ctypes.data_as(ctypes.c_void_p)),
+                   ctx, 'mftx_ptt_fit')
+        t = np.empty((nseg, 3))
+        for k, (g00, g01, g02, g11, g12, g22, b0, b1, b2) in enumerate(normal):
+            G = np.array([[g00, g01, g02], [g01, g11, g12], [g02, g12, g22]])
+            t[k] = np.linalg.lstsq(G, np.array([b0, b1, b2]), rcond=None)[0]
+        tilt = np.ascontiguousarray(t[:, 1:3])
+        out = torch.empty((rows, cols), dtype=torch.float64, device=opd_d.device)
+        _lib.check(lib.mftx_remove_tilt(ctx, ctypes.c_void_p(opd_d.data_ptr()), _lib.DT_F64, cols,
+                                        ctypes.c_void_p(mask_d.data_ptr()), _lib.DT_F64, cols, rows * cols, nseg,
+                                        1 if nseg > 1 else 0, rows, cols, float(dx[0]), float(dx[1]),
+                                        tilt.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(out.data_ptr()), cols),
+                   ctx, 'mftx_remove_tilt')
+        plane.opd = out.cpu().numpy()
+        plane.tilt.extend([Tilt(x=t[k, 1], y=t[k, 2]) for k in range(nseg)])
+        return plane
+
     def __mul__(self, wavefront):
         if not _can_mul_ptype(wavefront.ptype, self.ptype):
             raise TypeError(f"can't multiply Wavefront with ptype '{wavefront.ptype}' by Plane with ptype "

@@ -1,6 +1,7 @@
-"""Zernike polynomials for building OPD maps (lentil zernike.py, Noll order).
+"""Zernike polynomials (lentil zernike.py, Noll order).
 
-Host-side input generation (not part of the propagation hot path).
+``zernike``, ``zernike_compose`` and ``zernike_coordinates`` build OPD inputs
+on the host; ``zernike_basis`` evaluates a mode stack on the device.
 """
 from math import factorial
 
@@ -76,13 +77,37 @@ def zernike_compose(mask, coeffs, normalize=True, rho=None, theta=None):
     return opd
 
 
-def zernike_basis(mask, modes, vectorize=False, normalize=True, rho=None, theta=None):
-    """Stack of Zernike modes over a mask (zernike.py:183-234)."""
+def zernike_basis(mask, modes, vectorize=False, normalize=True, rho=None, theta=None, *, device=False):
+    """Stack of Noll-ordered Zernike modes over a mask (zernike.py:183-234),
+    evaluated on the device by ``mftx_zernike_basis`` (one thread per pixel
+    and mode, the radial polynomial summed in the reference's order).
+
+    NumPy float64 out by default; ``device=True`` returns the float64 device
+    tensor (no host copy), e.g. as the modal basis of a phase retrieval."""
+    import ctypes
+    import torch
+    from . import _lib
+    from ._arrays import to_dev
     modes = np.atleast_1d(np.asarray(modes))
     mask = np.asarray(mask)
     if rho is None:
         rho, theta = zernike_coordinates(mask)
-    basis = np.zeros(modes.shape + mask.shape)
-    for idx, j in np.ndenumerate(modes):
-        basis[idx] = zernike(mask, int(j), normalize, rho, theta)
-    return basis.reshape(basis.shape[0], -1) if vectorize else basis
+    elif theta is None:
+        raise ValueError('Both rho and theta must be specified')
+    rho = np.broadcast_to(np.asarray(rho, dtype=float), mask.shape)
+    theta = np.broadcast_to(np.asarray(theta, dtype=float), mask.shape)
+    nm = np.array([noll_to_nm(int(j)) for j in modes.ravel()], dtype=np.int32).reshape(-1, 2)
+    npix = int(np.prod(mask.shape))
+    rd = to_dev(np.ascontiguousarray(rho))
+    td = to_dev(np.ascontiguousarray(theta))
+    md = to_dev(np.ascontiguousarray(mask, dtype=np.float64))
+    out = torch.empty((nm.shape[0], npix), dtype=torch.float64, device=rd.device)
+    lib = _lib.load()
+    ctx = _lib.context()
+    _lib.check(lib.mftx_zernike_basis(ctx, ctypes.c_void_p(rd.data_ptr()), ctypes.c_void_p(td.data_ptr()),
+                                      ctypes.c_void_p(md.data_ptr()), _lib.DT_F64, npix,
+                                      nm.ctypes.data_as(ctypes.c_void_p), nm.shape[0], 1 if normalize else 0,
+                                      ctypes.c_void_p(out.data_ptr())), ctx, 'mftx_zernike_basis')
+    shape = (modes.shape[0], npix) if vectorize else modes.shape + mask.shape
+    out = out.reshape(shape)
+    return out if device else out.cpu().numpy()

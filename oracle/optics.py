@@ -486,6 +486,41 @@ def propagate_dft(w, pixelscale, shape=None, prop_shape=None, oversample=2, mask
     return out
 
 
+def ptt_vector(shape, pixelscale, mask):
+    """plane.py:521-561: rows [mask, mask * r * dx_r, -mask * c * dx_c] per
+    segment (3 * nseg rows) over the raveled grid; r, c from the centred mesh
+    (helper.py:7-17)."""
+    pixelscale = np.broadcast_to(pixelscale, (2,))
+    nr, nc = shape
+    r, c = np.meshgrid(np.arange(nr) - np.floor(nr / 2.0), np.arange(nc) - np.floor(nc / 2.0), indexing='ij')
+    unmasked = np.einsum('ij,i->ij', [np.ones(r.size), r.ravel(), -c.ravel()], [1, pixelscale[0], pixelscale[1]])
+    mask = np.asarray(mask)
+    if mask.ndim == 2:
+        return np.einsum('ij,j->ij', unmasked, mask.ravel())
+    out = np.empty((mask.shape[0] * 3, r.size))
+    for k in range(mask.shape[0]):
+        out[3 * k:3 * k + 3] = unmasked * mask[k].ravel()
+    return out
+
+
+def fit_tilt(opd, mask, pixelscale):
+    """plane.py:563-611: least-squares piston/tip/tilt per segment; returns
+    (opd with the tilt removed, [(x, y) tilt per segment])."""
+    opd = np.asarray(opd, dtype=float)
+    mask = np.asarray(mask)
+    v = ptt_vector(opd.shape, pixelscale, mask)
+    if mask.ndim == 2:
+        t = np.linalg.lstsq(v.T, opd.ravel(), rcond=None)[0]
+        return opd - np.einsum('ij,i->j', v[1:3], t[1:3]).reshape(opd.shape), [(t[1], t[2])]
+    tilts, parts = [], []
+    for k in range(mask.shape[0]):
+        t = np.linalg.lstsq(v[3 * k:3 * k + 3].T, opd.ravel(), rcond=None)[0]
+        seg = np.einsum('ij,i->j', v[3 * k + 1:3 * k + 3], t[1:3]).reshape(opd.shape)
+        parts.append((opd - seg) * mask[k])
+        tilts.append((t[1], t[2]))
+    return np.sum(parts, axis=0), tilts
+
+
 # --------------------------------------------------------------------------
 # FFT propagation (propagate.py:9-135)
 # --------------------------------------------------------------------------
