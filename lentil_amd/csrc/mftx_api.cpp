@@ -1166,6 +1166,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   std::vector<std::vector<Work>> clus1, clus2;
   struct Patch { int idx; size_t a, dst; };  // workspace offsets to rebase after allocation
   std::vector<Patch> patch1, patch2;
+  std::vector<int64_t> s1rows;  // tiled rows of each stage-1 problem
 
   for (auto& c : cls) {
     const ClassKey& K = c.key;
@@ -1203,7 +1204,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       const size_t elem = prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(uint16_t);
       patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)op.slot.at(P.wave) * packed_bytes(prec, op.rows, op.kpad),
                              c.abuf + (size_t)c.kbeg[j] * elem});
-      add_tiles(clus1, (int)p1.size(), (int)p1.size() + 1, op.rows, q.cols, tile_n);
+      s1rows.push_back(op.rows);
       p1.push_back(q);
     }
     // stage 2, one problem per class (K concatenated over members)
@@ -1238,6 +1239,24 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     p2.push_back(q);
   }
 
+  // stage-1 works: runs of up to g1 consecutive problems with one tiling and K
+  // length (the wavelengths of a tile) share a work item, so the staging
+  // pipeline runs on across their K loops instead of refilling for every
+  // short K loop; g1 keeps at least one work item per CU
+  {
+    int64_t single = 0;
+    for (size_t i = 0; i < p1.size(); ++i)
+      single += (s1rows[i] / BM) * ((p1[i].cols + slots_per_tile(tile_n) - 1) / slots_per_tile(tile_n));
+    const size_t g1 = (size_t)std::max<int64_t>(1, std::min<int64_t>(8, single / 256));
+    for (size_t i = 0; i < p1.size();) {
+      size_t e = i + 1;
+      while (e < p1.size() && e - i < g1 && s1rows[e] == s1rows[i] && p1[e].cols == p1[i].cols &&
+             p1[e].kpad == p1[i].kpad && p1[e].rows == p1[i].rows)
+        ++e;
+      add_tiles(clus1, (int)i, (int)e, s1rows[i], p1[i].cols, tile_n);
+      i = e;
+    }
+  }
   mftx_plan* plan = new mftx_plan();
   plan->ctx = ctx;
   plan->prec = prec;
