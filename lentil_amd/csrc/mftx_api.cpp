@@ -359,8 +359,8 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
       op.dst = nullptr;  // rebased below
       op.rows_pad = g.rows1;
       op.kpad = g.kpad1;
-      op.begin = pk_total;
-      pk_total += g.rows1 * (g.kpad1 / 16);
+      op.begin = group_begin(pk_total);
+      pk_total = op.begin + g.rows1 * (g.kpad1 / 2);  // one thread per (row, pair of K values)
       packs.push_back(op);
       wave_of.push_back(-(int)packs.size());  // -(pack index + 1)
       continue;
@@ -379,7 +379,8 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
                (long long)stride == last.wave_stride;
     }
     if (!joined) {
-      if (!phops.empty()) ph_total += phasor_threads(phops.back().rows_pad, phops.back().kpad, phops.back().nwave);
+      if (!phops.empty())
+        ph_total = phops.back().begin + phasor_threads(phops.back().rows_pad, phops.back().kpad, phops.back().nwave);
       PhasorOp op = {};
       op.amp = q.amp;
       op.opd = q.opd;
@@ -396,14 +397,15 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
       op.wave_stride = (long long)stride;
       op.rows_pad = g.rows1;
       op.kpad = g.kpad1;
-      op.begin = ph_total;
+      op.begin = group_begin(ph_total);
       phops.push_back(op);
     }
     phops.back().nwave += 1;
     waves.push_back(PhasorWave{q.inv_wl, g.g1.fold, g.x01});
     wave_of.push_back((int)waves.size() - 1);
   }
-  if (!phops.empty()) ph_total += phasor_threads(phops.back().rows_pad, phops.back().kpad, phops.back().nwave);
+  if (!phops.empty())
+    ph_total = phops.back().begin + phasor_threads(phops.back().rows_pad, phops.back().kpad, phops.back().nwave);
 
   std::vector<Problem> p1(count), p2(count);
   std::vector<double> cx;
@@ -428,7 +430,6 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
     a.ldd = packed_lda(prec, g.kpad2);
     a.fold = g.g2.fold;
     a.fold_x0 = g.x02;
-    add_tiles(clus1, i, i + 1, g.rows1, a.cols, bn);
 
     Problem& b = p2[i];
     b.lda = packed_lda(prec, g.kpad2);
@@ -449,8 +450,29 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
     b.wmap = q.w;
     b.ldw = q.ldw;
     b.wdtype = q.w_dtype;
-    add_tiles(clus2, i, i + 1, g.rows2, b.cols, bn);
   }
+  // works: runs of up to 8 consecutive entries with one tiling and K length
+  // share a work item (the staging pipeline runs across their K loops), as
+  // long as every CU keeps a work item
+  auto group_works = [&](std::vector<std::vector<Work>>& clus, const std::vector<Problem>& ps, bool first) {
+    int64_t single = 0;
+    for (int32_t i = 0; i < count; ++i) {
+      const int64_t rows = first ? geo[i].rows1 : geo[i].rows2;
+      single += (rows / BM) * ((ps[i].cols + slots_per_tile(bn) - 1) / slots_per_tile(bn));
+    }
+    const int32_t gmax = (int32_t)std::max<int64_t>(1, std::min<int64_t>(8, single / 256));
+    for (int32_t i = 0; i < count;) {
+      const int64_t rows = first ? geo[i].rows1 : geo[i].rows2;
+      int32_t e = i + 1;
+      while (e < count && e - i < gmax && (first ? geo[e].rows1 : geo[e].rows2) == rows && ps[e].cols == ps[i].cols &&
+             ps[e].kpad == ps[i].kpad && ps[e].rows == ps[i].rows)
+        ++e;
+      add_tiles(clus, i, e, rows, ps[i].cols, bn);
+      i = e;
+    }
+  };
+  group_works(clus1, p1, true);
+  group_works(clus2, p2, false);
   const std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
 
   // one descriptor block: waves, phasor ops, pack ops, problems, chunk origins, works
@@ -1399,8 +1421,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     ph.wave_stride = (long long)packed_bytes(prec, op.rows, op.kpad);
     ph.rows_pad = op.rows;
     ph.kpad = op.kpad;
-    ph.begin = pthreads;
-    pthreads += phasor_threads(ph.rows_pad, ph.kpad, ph.nwave);
+    ph.begin = group_begin(pthreads);
+    pthreads = ph.begin + phasor_threads(ph.rows_pad, ph.kpad, ph.nwave);
     ops.push_back(ph);
   }
   plan->phasor_ops = (PhasorOp*)(base + o_ops);

@@ -120,8 +120,14 @@ __host__ __device__ inline long long phasor_threads(long long rows_pad, int kpad
   return rows_pad * (kpad / 16) * ((nwave + PH_WAVES - 1) / PH_WAVES);
 }
 
+// Grouped launches (phasor and pack ops): blocks of kGroupBlock threads, and
+// every op's first thread is a multiple of kGroupBlock, so a block belongs to
+// one op and finds it with a block-uniform search.
+constexpr int kGroupBlock = 256;
+inline long long group_begin(long long total) { return (total + kGroupBlock - 1) / kGroupBlock * kGroupBlock; }
+
 // One packed operand from complex data (mftx_dft2_many): src (or src^T) times
-// the fold cis(fold * (x0 + k)) along K.  `begin` is the op's first thread.
+// the fold cis(fold * (x0 + k)) along K; rows_pad * kpad / 2 threads from `begin`.
 struct PackOp {
   const void* src;
   int dtype, trans;

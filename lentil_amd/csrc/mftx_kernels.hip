@@ -83,26 +83,36 @@ hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, lo
   return hipGetLastError();
 }
 
-// Grouped form of pack_kernel: one thread per (op, row, 16-wide K block); the
-// op is found by binary search on its first thread.
-__global__ void pack_group_kernel(int prec, const PackOp* __restrict__ ops, int nops, long long total) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
+// Index of the op that block b0 (its first thread) belongs to: ops start on
+// block boundaries, so the search runs on block-uniform values (scalar loads).
+template <typename Op>
+__device__ __forceinline__ int block_op(const Op* __restrict__ ops, int nops, long long b0) {
   int lo = 0, hi = nops - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (ops[mid].begin <= t) lo = mid;
+    if (ops[mid].begin <= b0) lo = mid;
     else hi = mid - 1;
   }
-  const PackOp op = ops[lo];
-  const long long u = t - op.begin;
-  const int kblocks = op.kpad / 16;
-  const long long row = u / kblocks;
-  const int kb = (int)(u % kblocks);
-  float2 v[16];
+  return lo;
+}
+
+// Grouped form of pack_kernel with one thread per (op, row, pair of K
+// values): lanes run along K (along the packed row) for a row-major source and
+// along the packed rows (along the source row) for a transposed one, so the
+// source reads of a wave are contiguous; each thread writes 4 bytes of every
+// BF16 plane (or one 16-byte complex64 pair).
+__global__ void pack_group_kernel(int prec, const PackOp* __restrict__ ops, int nops, long long total) {
+  const long long b0 = (long long)blockIdx.x * kGroupBlock;
+  const PackOp& op = ops[block_op(ops, nops, b0)];
+  const long long u = b0 + threadIdx.x - op.begin;
+  const int npairs = op.kpad / 2;
+  if (u >= op.rows_pad * npairs) return;
+  const long long row = op.trans ? u % op.rows_pad : u / npairs;
+  const int kp = (int)(op.trans ? u / op.rows_pad : u % npairs);
+  float2 v[2];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const long long k = (long long)kb * 16 + i;
+  for (int i = 0; i < 2; ++i) {
+    const long long k = 2LL * kp + i;
     const long long r = op.trans ? k : row, c = op.trans ? row : k;
     if (r < op.m && c < op.n) {
       const float2 e = load_elem(op.src, op.dtype, r * op.lds + c);
@@ -111,14 +121,26 @@ __global__ void pack_group_kernel(int prec, const PackOp* __restrict__ ops, int 
       v[i] = make_float2(0.f, 0.f);
     }
   }
-  store_packed16(prec, op.dst, packed_lda(prec, op.kpad), row, kb, v);
+  const long long lda = packed_lda(prec, op.kpad);
+  const int k0 = 2 * kp;
+  if (prec == PREC_BF16X3) {
+    uint32_t rh, rl, ih, il;
+    split2(v[0].x, v[1].x, rh, rl);
+    split2(v[0].y, v[1].y, ih, il);
+    uint32_t* d = (uint32_t*)((__bf16*)op.dst + row * lda + (long long)(k0 / 16) * 64 + (k0 % 16));
+    d[0] = rh;    // plane 0 (re hi) at +0, plane p at + p * 16 bf16 = 8 words
+    d[8] = rl;
+    d[16] = ih;
+    d[24] = il;
+  } else {
+    *(float4*)((float2*)op.dst + row * lda + k0) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+  }
 }
 
 hipError_t launch_pack_group(int prec, const PackOp* ops, int nops, long long total, hipStream_t s) {
   if (total <= 0 || nops <= 0) return hipSuccess;
-  const int bs = 256;
-  hipLaunchKernelGGL(pack_group_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, ops, nops,
-                     total);
+  hipLaunchKernelGGL(pack_group_kernel, dim3((unsigned)((total + kGroupBlock - 1) / kGroupBlock)), dim3(kGroupBlock),
+                     0, s, prec, ops, nops, total);
   return hipGetLastError();
 }
 
@@ -139,16 +161,10 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
 // take consecutive source columns (K blocks when not trans, rows of the
 // transposed operand when trans), so the source reads of a wave are contiguous.
 __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, int nops, long long total) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  int lo = 0, hi = nops - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ops[mid].begin <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  const PhasorOp op = ops[lo];
-  const long long u = t - op.begin;
+  const long long b0 = (long long)blockIdx.x * kGroupBlock;
+  const PhasorOp& op = ops[block_op(ops, nops, b0)];
+  const long long u = b0 + threadIdx.x - op.begin;
+  if (u >= phasor_threads(op.rows_pad, op.kpad, op.nwave)) return;
   const int kblocks = op.kpad / 16;
   const long long per_group = op.rows_pad * kblocks;
   const int g = (int)(u / per_group);
@@ -182,9 +198,8 @@ __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, 
 
 hipError_t launch_phasor_group(int prec, const PhasorOp* ops, int nops, long long total, hipStream_t s) {
   if (total <= 0 || nops <= 0) return hipSuccess;
-  const int bs = 256;
-  hipLaunchKernelGGL(phasor_group_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, ops, nops,
-                     total);
+  hipLaunchKernelGGL(phasor_group_kernel, dim3((unsigned)((total + kGroupBlock - 1) / kGroupBlock)),
+                     dim3(kGroupBlock), 0, s, prec, ops, nops, total);
   return hipGetLastError();
 }
 
