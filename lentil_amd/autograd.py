@@ -14,7 +14,7 @@ fields (nothing is stored but the inputs) and applies the adjoint MFT:
     D = dL/d conj(f) = dft2_adjoint(w g F),   g = dL/dI
     dL/d opd = sum (2 pi / l) (-2) Im(conj(D) f),  dL/d amp = sum 2 Re(conj(D) e^{i theta})
 
-The forward runs as one batched PSFPlan per diversity; the backward is two
+The forward runs as one batched PSFPlan over every diversity; the backward is two
 grouped MFT calls (mftx_dft2_many: the weighted forward fields, then the
 adjoint with conjugate twiddles) and one mftx_phase_grad_many per diversity.
 """
@@ -36,7 +36,7 @@ class BroadbandPSFModel:
     """Multi-diversity broadband PSF model on the device with gradients."""
 
     def __init__(self, amp, diversity, wavelengths, weights, dx, du, z, npix, oversample=2,
-                 precision='bf16x3', lambda_group=4):
+                 precision='bf16x3', lambda_group=8):
         self.dev = _lib.device()
         amp = np.asarray(amp, dtype=float)
         self.shape = amp.shape
@@ -51,9 +51,12 @@ class BroadbandPSFModel:
         # a full-grid mask keeps the tile the whole array at offset (0, 0)
         pupil = Pupil(amplitude=amp, opd=np.zeros(self.shape), mask=np.ones(self.shape), pixelscale=dx,
                       focal_length=z)
-        self.plans = [PSFPlan(pupil, self.wavelengths, self.weights, pixelscale=du, shape=npix,
-                              oversample=oversample, precision=precision, lambda_group=lambda_group)
-                      for _ in self.div]
+        # one plan for every diversity: the diversity images lie side by side
+        # in one detector, so each launch covers all of them
+        self.plan = PSFPlan(pupil, self.wavelengths, self.weights, pixelscale=du, shape=npix,
+                            oversample=oversample, precision=precision, lambda_group=lambda_group,
+                            copies=len(self.div))
+        self._wide = torch.empty(self.plan.shape_out, dtype=torch.float32, device=self.dev)
         self.alphas = [(self.dx * self.du) / (wl * self.z * self.oversample) for wl in self.wavelengths]
 
     def __call__(self, opd, amp=None):
@@ -64,12 +67,11 @@ class BroadbandPSFModel:
 
     # ---------------------------------------------------------------- forward
     def forward_images(self, opd, amp):
-        out = torch.empty((len(self.div), self.M, self.M), dtype=torch.float32, device=self.dev)
-        for d, (div, plan) in enumerate(zip(self.div, self.plans)):
-            plan.set_opd(0, opd + div)
-            plan.set_amplitude(0, amp)
-            plan(out[d])
-        return out
+        for d, div in enumerate(self.div):
+            self.plan.set_opd(0, opd + div, copy=d)
+            self.plan.set_amplitude(0, amp, copy=d)
+        self.plan(self._wide)
+        return self.plan.images(self._wide).contiguous()
 
     # --------------------------------------------------------------- backward
     def backward_grads(self, opd, amp, grad_images, need_amp):

@@ -142,13 +142,30 @@ class PSFPlan:
     128 columns, 0 = library default); ``autotune`` times every
     (lambda_group, tile_n) candidate on this device and keeps the fastest
     (remembered per problem shape for the process); ``use_graph`` replays the
-    launches from a hipGraph after the first call.
+    launches from a hipGraph after the first call.  ``copies`` > 1 makes that
+    many independent copies of the pupil's tiles (each with its own device
+    amplitude and OPD, see ``set_opd``) whose images lie side by side in one
+    (rows, copies * cols) detector, all computed by the same launches.
     """
 
     def __init__(self, pupil, wavelengths, weights=None, pixelscale=None, shape=None, prop_shape=None,
                  oversample=2, mask=None, tilts=(), precision='bf16x3', lambda_group=8, tile_n=0,
-                 use_graph=True, autotune=False):
+                 use_graph=True, autotune=False, copies=1):
         geo = plan_geometry(pupil, wavelengths, weights, pixelscale, shape, prop_shape, oversample, mask, tilts)
+        copies = int(copies)
+        if copies < 1:
+            raise ValueError('copies must be positive')
+        self.copies = copies
+        self.image_shape = geo['shape_out']
+        ntile0, ngroup0 = len(geo['tiles']), 1 + max((p['group'] for p in geo['problems']), default=-1)
+        if copies > 1:
+            width = geo['shape_out'][1]
+            geo['tiles'] = geo['tiles'] * copies
+            geo['problems'] = [dict(p, tile=p['tile'] + k * ntile0, group=p['group'] + k * ngroup0,
+                                    out_c0=p['out_c0'] + k * width)
+                               for k in range(copies) for p in geo['problems']]
+            geo['shape_out'] = (geo['shape_out'][0], copies * width)
+        self._ntile0 = ntile0
         self.wavelengths = geo['wavelengths']
         self.weights = geo['weights']
         self.precision = precision
@@ -256,13 +273,19 @@ class PSFPlan:
         _lib.check(rc, ctx, 'mftx_plan_profile')
         return dict(phasor=ms[0], stage1=ms[1], stage2=ms[2], reduce=ms[3])
 
-    def set_opd(self, tile, opd):
-        """Overwrite a tile's OPD in place (the plan re-reads it on every call)."""
-        self._amp_opd[tile][1].copy_(torch.as_tensor(opd, dtype=torch.float64))
+    def set_opd(self, tile, opd, copy=0):
+        """Overwrite a tile's OPD (of one copy) in place (the plan re-reads it on every call)."""
+        self._amp_opd[copy * self._ntile0 + tile][1].copy_(torch.as_tensor(opd, dtype=torch.float64))
 
-    def set_amplitude(self, tile, amp):
-        """Overwrite a tile's (masked) amplitude in place."""
-        self._amp_opd[tile][0].copy_(torch.as_tensor(amp, dtype=torch.float32))
+    def set_amplitude(self, tile, amp, copy=0):
+        """Overwrite a tile's (masked) amplitude (of one copy) in place."""
+        self._amp_opd[copy * self._ntile0 + tile][0].copy_(torch.as_tensor(amp, dtype=torch.float32))
+
+    def images(self, out):
+        """The per-copy images of a (rows, copies * cols) result, as a
+        (copies, rows, cols) view."""
+        rows, cols = self.image_shape
+        return out.view(rows, self.copies, cols).permute(1, 0, 2)
 
     def close(self):
         if getattr(self, '_plan', None):

@@ -332,3 +332,32 @@ def test_autotune(L):
     again = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, autotune=True)
     assert not hasattr(again, 'tuning')
     assert (again.lambda_group, again.tile_n) == (plan.lambda_group, plan.tile_n)
+
+
+def test_plan_copies(L):
+    """copies=3: three independent pupils (different OPDs) side by side in one
+    detector equal three separate plans, image for image."""
+    from lentil_amd import workloads
+    cfg = workloads.mono_256()
+    wl = np.array([600e-9, 650e-9, 700e-9])
+    wt = np.array([0.2, 0.5, 0.3])
+    pupil = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'], focal_length=cfg['focal_length'])
+    kw = dict(pixelscale=cfg['pixelscale'], shape=cfg['shape'], oversample=cfg['oversample'], precision='fp32',
+              lambda_group=4)
+    multi = L.PSFPlan(pupil, wl, wt, copies=3, **kw)
+    rows, cols = multi.image_shape
+    assert multi.shape_out == (rows, 3 * cols)
+    rng = np.random.default_rng(21)
+    opds = [cfg['opd'] + 1e-8 * k * rng.standard_normal(cfg['opd'].shape) for k in range(3)]
+    for k, o in enumerate(opds):
+        multi.set_opd(0, o, copy=k)
+    wide = multi()
+    imgs = multi.images(wide)
+    assert imgs.shape == (3, rows, cols)
+    for k, o in enumerate(opds):
+        single = L.PSFPlan(L.Pupil(amplitude=cfg['amp'], opd=o, pixelscale=cfg['dx'],
+                                   focal_length=cfg['focal_length']), wl, wt, **kw)
+        ref = single().double().cpu().numpy()
+        assert rel_l2(imgs[k].double().cpu().numpy(), ref) < 1e-6
+        single.close()
+    multi.close()
