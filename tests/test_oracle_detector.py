@@ -107,3 +107,21 @@ def test_jitter_fwhm_known_answer():
     arr[arr < 0.5] = 0
     rows, cols = np.flatnonzero(arr.any(1)), np.flatnonzero(arr.any(0))
     assert max(rows[-1] - rows[0], cols[-1] - cols[0]) // 2 == 2
+
+
+def test_host_errors_before_device_work():
+    """The drop-in layer raises the reference's exception types before any
+    device work (util.py:243-244 for complex rebin data; this build's device
+    rescale covers mode 'nearest' and orders 0/1/3 only)."""
+    from lentil_amd import util
+    with pytest.raises(ValueError):
+        util.rebin(np.ones((4, 4)) * 1j, 2)
+    with pytest.raises(NotImplementedError):
+        util.rescale(np.ones((4, 4)), 0.5, mode='wrap')
+    with pytest.raises(NotImplementedError):
+        util.rescale(np.ones((4, 4)), 0.5, order=2)
+    with pytest.raises(NotImplementedError):
+        util.rescale(np.ones((4, 4)) * 1j, 0.5)
+    from lentil_amd import detector
+    with pytest.raises(TypeError):
+        detector.pixel(np.ones((4, 4)) * 1j)
