@@ -283,6 +283,36 @@ static int run_single(mftx_ctx* ctx, int prec, int epi, int bn, Problem p, doubl
   return MFTX_OK;
 }
 
+// Problem ranges [b, e) for work items: runs of g consecutive problems with
+// one tiling (rows tiled, output slots) and K length share a work item, so
+// the GEMM's staging pipeline runs across their K loops.  g is at most 8 and
+// keeps at least one work item per CU.  Grouping is used only when every run
+// has the same length g: a launch that mixed long and short work items would
+// finish on its long tail.
+static std::vector<std::pair<int, int>> group_runs(const std::vector<Problem>& ps, const std::vector<int64_t>& rows,
+                                                   int bn) {
+  const int n = (int)ps.size();
+  int64_t single = 0;
+  for (int i = 0; i < n; ++i) single += (rows[i] / BM) * ((ps[i].cols + slots_per_tile(bn) - 1) / slots_per_tile(bn));
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(8, single / 256));
+  std::vector<std::pair<int, int>> runs;
+  bool uniform = true;
+  for (int i = 0; i < n;) {
+    int e = i + 1;
+    while (e < n && e - i < g && rows[e] == rows[i] && ps[e].cols == ps[i].cols && ps[e].kpad == ps[i].kpad &&
+           ps[e].rows == ps[i].rows)
+      ++e;
+    uniform = uniform && e - i == g;
+    runs.emplace_back(i, e);
+    i = e;
+  }
+  if (!uniform) {
+    runs.clear();
+    for (int i = 0; i < n; ++i) runs.emplace_back(i, i + 1);
+  }
+  return runs;
+}
+
 // Geometry of one entry of mftx_dft2_many (the quantities mftx_dft2 derives).
 namespace {
 struct ManyGeo {
@@ -451,25 +481,11 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
     b.ldw = q.ldw;
     b.wdtype = q.w_dtype;
   }
-  // works: runs of up to 8 consecutive entries with one tiling and K length
-  // share a work item (the staging pipeline runs across their K loops), as
-  // long as every CU keeps a work item
+  // works: runs of consecutive entries with one geometry share a work item
   auto group_works = [&](std::vector<std::vector<Work>>& clus, const std::vector<Problem>& ps, bool first) {
-    int64_t single = 0;
-    for (int32_t i = 0; i < count; ++i) {
-      const int64_t rows = first ? geo[i].rows1 : geo[i].rows2;
-      single += (rows / BM) * ((ps[i].cols + slots_per_tile(bn) - 1) / slots_per_tile(bn));
-    }
-    const int32_t gmax = (int32_t)std::max<int64_t>(1, std::min<int64_t>(8, single / 256));
-    for (int32_t i = 0; i < count;) {
-      const int64_t rows = first ? geo[i].rows1 : geo[i].rows2;
-      int32_t e = i + 1;
-      while (e < count && e - i < gmax && (first ? geo[e].rows1 : geo[e].rows2) == rows && ps[e].cols == ps[i].cols &&
-             ps[e].kpad == ps[i].kpad && ps[e].rows == ps[i].rows)
-        ++e;
-      add_tiles(clus, i, e, rows, ps[i].cols, bn);
-      i = e;
-    }
+    std::vector<int64_t> rows(count);
+    for (int32_t i = 0; i < count; ++i) rows[i] = first ? geo[i].rows1 : geo[i].rows2;
+    for (const auto& r : group_runs(ps, rows, bn)) add_tiles(clus, r.first, r.second, rows[r.first], ps[r.first].cols, bn);
   };
   group_works(clus1, p1, true);
   group_works(clus2, p2, false);
@@ -1261,23 +1277,14 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     p2.push_back(q);
   }
 
-  // stage-1 works: runs of up to g1 consecutive problems with one tiling and K
+  // stage-1 works: runs of g1 consecutive problems with one tiling and K
   // length (the wavelengths of a tile) share a work item, so the staging
   // pipeline runs on across their K loops instead of refilling for every
   // short K loop; g1 keeps at least one work item per CU
   {
-    int64_t single = 0;
-    for (size_t i = 0; i < p1.size(); ++i)
-      single += (s1rows[i] / BM) * ((p1[i].cols + slots_per_tile(tile_n) - 1) / slots_per_tile(tile_n));
-    const size_t g1 = (size_t)std::max<int64_t>(1, std::min<int64_t>(8, single / 256));
-    for (size_t i = 0; i < p1.size();) {
-      size_t e = i + 1;
-      while (e < p1.size() && e - i < g1 && s1rows[e] == s1rows[i] && p1[e].cols == p1[i].cols &&
-             p1[e].kpad == p1[i].kpad && p1[e].rows == p1[i].rows)
-        ++e;
-      add_tiles(clus1, (int)i, (int)e, s1rows[i], p1[i].cols, tile_n);
-      i = e;
-    }
+    std::vector<int64_t> rows(s1rows.begin(), s1rows.end());
+    for (const auto& r : group_runs(p1, rows, tile_n)) add_tiles(clus1, r.first, r.second, s1rows[r.first],
+                                                                 p1[r.first].cols, tile_n);
   }
   mftx_plan* plan = new mftx_plan();
   plan->ctx = ctx;
