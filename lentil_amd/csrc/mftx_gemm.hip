@@ -276,6 +276,62 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
   }
   const float sc = P.scale;
   const bool has_p = jp < N, has_m = jm >= 0 && jm < N && jm != jp;
+  if (EPI == EPI_STORE_PACKED_T && PREC == PREC_BF16X3) {
+    // Lane l holds K 0-3 (group 2h) and 8-11 (group 2h+1) of a 16-K block of
+    // its output row, lane l+32 K 4-7 and 12-15.  One lane swap per word
+    // (v_permlane32_swap: the high half of the first operand trades places
+    // with the low half of the second) gives lane l K 0-7 and lane l+32 K
+    // 8-15, so each lane stores 16 bytes of every plane.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int kb = wk.tr * BM + wr * 32 + 16 * h;  // first K of the block (wave-uniform)
+      if (kb >= P.limit) continue;
+      uint32_t w[2][2][4][2];  // [group of the pair][side][plane][word]
+#pragma unroll
+      for (int gg = 0; gg < 2; ++gg) {
+        const int g = 2 * h + gg;
+        const int row0 = wk.tr * BM + wr * 32 + 8 * g + 4 * (lane >> 5);
+        float v[2][2][4];  // [side][re, im][e]
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float pr = acc[0][4 * g + e], pi = acc[1][4 * g + e], qr = acc[2][4 * g + e], qi = acc[3][4 * g + e];
+          const float2 f = cis_turns(P.fold * (P.fold_x0 + (double)(row0 + e)));
+          const float ar = (pr - qi) * sc, ai = (pi + qr) * sc, br = (pr + qi) * sc, bi = (pi - qr) * sc;
+          v[0][0][e] = ar * f.x - ai * f.y;
+          v[0][1][e] = ar * f.y + ai * f.x;
+          v[1][0][e] = br * f.x - bi * f.y;
+          v[1][1][e] = br * f.y + bi * f.x;
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          split2(v[side][0][0], v[side][0][1], w[gg][side][0][0], w[gg][side][1][0]);
+          split2(v[side][0][2], v[side][0][3], w[gg][side][0][1], w[gg][side][1][1]);
+          split2(v[side][1][0], v[side][1][1], w[gg][side][2][0], w[gg][side][3][0]);
+          split2(v[side][1][2], v[side][1][3], w[gg][side][2][1], w[gg][side][3][1]);
+        }
+      }
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        uint32_t o[4][4];
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const auto r = __builtin_amdgcn_permlane32_swap(w[0][side][pl][q], w[1][side][pl][q], false, false);
+            o[pl][q] = r[0];
+            o[pl][2 + q] = r[1];
+          }
+        }
+        if (side ? has_m : has_p) {
+          __bf16* d = (__bf16*)P.dst + (long long)(side ? jm : jp) * P.ldd + (long long)(kb >> 4) * 64 + 8 * (lane >> 5);
+#pragma unroll
+          for (int pl = 0; pl < 4; ++pl)
+            *(uint4*)(d + 16 * pl) = make_uint4(o[pl][0], o[pl][1], o[pl][2], o[pl][3]);
+        }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     // rows row0 .. row0+3 of C in this lane
