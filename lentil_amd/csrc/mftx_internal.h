@@ -117,7 +117,7 @@ struct PhasorOp {
   long long begin;
 };
 __host__ __device__ inline long long phasor_threads(long long rows_pad, int kpad, int nwave) {
-  return rows_pad * (kpad / 16) * ((nwave + PH_WAVES - 1) / PH_WAVES);
+  return rows_pad * (kpad / 8) * ((nwave + PH_WAVES - 1) / PH_WAVES);
 }
 
 // Grouped launches (phasor and pack ops): blocks of kGroupBlock threads, and

@@ -152,30 +152,32 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
   return dtype == 3 ? ((const double*)p)[idx] : (double)((const float*)p)[idx];
 }
 
-// One thread per (op, row, 16-wide K block, group of PH_WAVES wavelengths),
-// every phasor operand of a plan in one launch (the op is found by binary
-// search on its first thread).  The 16 amp/opd values are read once and the
+// One thread per (op, row, half of a 16-wide K block, group of PH_WAVES
+// wavelengths), every phasor operand of a plan in one launch (the op is found
+// by a block-uniform search).  The 8 amp/opd values are read once and the
 // packed operand of the m x n tile (n x m when trans) is written for each
 // wavelength of the group, with that wavelength's stage-1 fold along K in the
 // same phase:  amp * cis(opd * inv_wl + fold * (x0 + k)).  Consecutive threads
-// take consecutive source columns (K blocks when not trans, rows of the
-// transposed operand when trans), so the source reads of a wave are contiguous.
+// take consecutive source columns (half blocks when not trans, rows of the
+// transposed operand when trans), so the source reads of a wave are
+// contiguous; the two halves of a block store one 32-byte plane together.
 __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, int nops, long long total) {
   const long long b0 = (long long)blockIdx.x * kGroupBlock;
   const PhasorOp& op = ops[block_op(ops, nops, b0)];
   const long long u = b0 + threadIdx.x - op.begin;
   if (u >= phasor_threads(op.rows_pad, op.kpad, op.nwave)) return;
-  const int kblocks = op.kpad / 16;
-  const long long per_group = op.rows_pad * kblocks;
+  const int khalves = op.kpad / 8;
+  const long long per_group = op.rows_pad * khalves;
   const int g = (int)(u / per_group);
   const long long r = u % per_group;
-  const long long row = op.trans ? r % op.rows_pad : r / kblocks;
-  const int kb = (int)(op.trans ? r / op.rows_pad : r % kblocks);
-  float a[16];
-  double o[16];
+  const long long row = op.trans ? r % op.rows_pad : r / khalves;
+  const int kh = (int)(op.trans ? r / op.rows_pad : r % khalves);
+  const int kb = kh >> 1, i0 = (kh & 1) * 8;
+  float a[8];
+  double o[8];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const long long k = (long long)kb * 16 + i;
+  for (int i = 0; i < 8; ++i) {
+    const long long k = (long long)kb * 16 + i0 + i;
     const long long sr = op.trans ? k : row, sc = op.trans ? row : k;
     const bool in = sr < op.m && sc < op.n;
     a[i] = in ? (float)load_real(op.amp, op.amp_dtype, sr * op.ld_amp + sc) : 0.f;
@@ -186,13 +188,30 @@ __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, 
   for (int w = g * PH_WAVES; w < w_end; ++w) {
     const PhasorWave pw = op.waves[w];
     const double f0 = pw.fold * (pw.x0 + (double)kb * 16);
-    float2 v[16];
+    float2 v[8];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const float2 c = cis_turns(__builtin_fma(o[i], pw.inv_wl, __builtin_fma(pw.fold, (double)i, f0)));
+    for (int i = 0; i < 8; ++i) {
+      const float2 c = cis_turns(__builtin_fma(o[i], pw.inv_wl, __builtin_fma(pw.fold, (double)(i0 + i), f0)));
       v[i] = make_float2(a[i] * c.x, a[i] * c.y);
     }
-    store_packed16(prec, (char*)op.dst + (long long)w * op.wave_stride, lda, row, kb, v);
+    char* dst = (char*)op.dst + (long long)w * op.wave_stride;
+    if (prec == PREC_BF16X3) {
+      uint32_t rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        split2(v[2 * i].x, v[2 * i + 1].x, rh[i], rl[i]);
+        split2(v[2 * i].y, v[2 * i + 1].y, ih[i], il[i]);
+      }
+      uint4* d = (uint4*)((__bf16*)dst + row * lda + (long long)kb * 64 + i0);  // plane p at + 16 p bf16
+      d[0] = make_uint4(rh[0], rh[1], rh[2], rh[3]);
+      d[2] = make_uint4(rl[0], rl[1], rl[2], rl[3]);
+      d[4] = make_uint4(ih[0], ih[1], ih[2], ih[3]);
+      d[6] = make_uint4(il[0], il[1], il[2], il[3]);
+    } else {
+      float4* d = (float4*)((float2*)dst + row * lda + (long long)kb * 16 + i0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[i] = make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+    }
   }
 }
 
