@@ -74,6 +74,43 @@ class BroadbandPSFModel:
         return self.plan.images(self._wide).contiguous()
 
     # --------------------------------------------------------------- backward
+    def _backward_state(self, m, n, M):
+        """Persistent device buffers of the backward pass and the two MFT lists
+        that point at them (built once: a call only refreshes the buffers)."""
+        st = getattr(self, '_bw', None)
+        if st is not None:
+            return st
+        nd, nw = len(self.div), len(self.wavelengths)
+        dev = self.dev
+        st = dict(amp=torch.empty((m, n), dtype=torch.float64, device=dev),
+                  tots=torch.empty((nd, m, n), dtype=torch.float64, device=dev),
+                  gI=torch.empty((nd, M, M), dtype=torch.float32, device=dev),
+                  Y=torch.empty((nd * nw, M, M), dtype=torch.complex64, device=dev),
+                  D=torch.empty((nd * nw, m, n), dtype=torch.complex64, device=dev))
+        fwd = (_lib.MftxMft * (nd * nw))()
+        adj = (_lib.MftxMft * (nd * nw))()
+        for d in range(nd):
+            for j, (wl, wt, a) in enumerate(zip(self.wavelengths, self.weights, self.alphas)):
+                i = d * nw + j
+                unit = float(abs(a))                      # sqrt(|a * a|), the unitary scale
+                q = fwd[i]
+                q.amp, q.amp_dtype, q.ld_amp = st['amp'].data_ptr(), _lib.DT_F64, n
+                q.opd, q.opd_dtype, q.ld_opd = st['tots'][d].data_ptr(), _lib.DT_F64, n
+                q.m, q.n, q.inv_wl = m, n, 1.0 / float(wl)
+                q.F, q.M, q.N, q.ldF = st['Y'][i].data_ptr(), M, M, M
+                q.alpha_r = q.alpha_c = a
+                q.scale = unit * float(wt)
+                q.w, q.w_dtype, q.ldw = st['gI'][d].data_ptr(), _lib.DT_F32, M
+                r = adj[i]
+                r.f, r.f_dtype, r.m, r.n, r.ldf = st['Y'][i].data_ptr(), _lib.DT_C64, M, M, M
+                r.F, r.M, r.N, r.ldF = st['D'][i].data_ptr(), m, n, n
+                r.alpha_r = r.alpha_c = a
+                r.scale, r.conj = unit, 1
+        st['fwd'], st['adj'] = fwd, adj
+        st['wls'] = (ctypes.c_double * nw)(*[float(w) for w in self.wavelengths])
+        self._bw = st
+        return st
+
     def backward_grads(self, opd, amp, grad_images, need_amp):
         """Gradients of sum_d <g_d, I_d> with respect to opd (and amp).
 
@@ -88,39 +125,19 @@ class BroadbandPSFModel:
         M = self.M
         nd, nw = len(self.div), len(self.wavelengths)
         prec = _lib.precision_code(self.precision)
-        amp64 = amp.detach().to(torch.float64).contiguous()
+        st = self._backward_state(m, n, M)
+        st['amp'].copy_(amp.detach())
+        for d, div in enumerate(self.div):
+            torch.add(opd.detach(), div, out=st['tots'][d])
+        st['gI'].copy_(grad_images.detach())
         g_opd = torch.zeros((m, n), dtype=torch.float64, device=self.dev)
         g_amp = torch.zeros((m, n), dtype=torch.float64, device=self.dev) if need_amp else None
-        Y = torch.empty((nd * nw, M, M), dtype=torch.complex64, device=self.dev)
-        D = torch.empty((nd * nw, m, n), dtype=torch.complex64, device=self.dev)
-        gI = grad_images.detach().contiguous()
-        tots = [(opd.detach() + div).contiguous() for div in self.div]
-        fwd = (_lib.MftxMft * (nd * nw))()
-        adj = (_lib.MftxMft * (nd * nw))()
+        _lib.check(lib.mftx_dft2_many(ctx, st['fwd'], nd * nw, prec), ctx, 'mftx_dft2_many (weighted forward)')
+        _lib.check(lib.mftx_dft2_many(ctx, st['adj'], nd * nw, prec), ctx, 'mftx_dft2_many (adjoint)')
         for d in range(nd):
-            for j, (wl, wt, a) in enumerate(zip(self.wavelengths, self.weights, self.alphas)):
-                i = d * nw + j
-                unit = float(abs(a))                      # sqrt(|a * a|), the unitary scale
-                q = fwd[i]
-                q.amp, q.amp_dtype, q.ld_amp = amp64.data_ptr(), _lib.DT_F64, n
-                q.opd, q.opd_dtype, q.ld_opd = tots[d].data_ptr(), _lib.DT_F64, n
-                q.m, q.n, q.inv_wl = m, n, 1.0 / float(wl)
-                q.F, q.M, q.N, q.ldF = Y[i].data_ptr(), M, M, M
-                q.alpha_r = q.alpha_c = a
-                q.scale = unit * float(wt)
-                q.w, q.w_dtype, q.ldw = gI[d].data_ptr(), _lib.dtype_code(gI[d]), gI.stride(1)
-                r = adj[i]
-                r.f, r.f_dtype, r.m, r.n, r.ldf = Y[i].data_ptr(), _lib.DT_C64, M, M, M
-                r.F, r.M, r.N, r.ldF = D[i].data_ptr(), m, n, n
-                r.alpha_r = r.alpha_c = a
-                r.scale, r.conj = unit, 1
-        _lib.check(lib.mftx_dft2_many(ctx, fwd, nd * nw, prec), ctx, 'mftx_dft2_many (weighted forward)')
-        _lib.check(lib.mftx_dft2_many(ctx, adj, nd * nw, prec), ctx, 'mftx_dft2_many (adjoint)')
-        wls = (ctypes.c_double * nw)(*[float(w) for w in self.wavelengths])
-        for d in range(nd):
-            _lib.check(lib.mftx_phase_grad_many(ctx, _ptr(amp64), _lib.DT_F64, n, _ptr(tots[d]), _lib.DT_F64, n, m, n,
-                                                wls, nw, _ptr(D[d * nw]), n, m * n, _ptr(g_opd), n,
-                                                _ptr(g_amp) if need_amp else None, n),
+            _lib.check(lib.mftx_phase_grad_many(ctx, _ptr(st['amp']), _lib.DT_F64, n, _ptr(st['tots'][d]),
+                                                _lib.DT_F64, n, m, n, st['wls'], nw, _ptr(st['D'][d * nw]), n, m * n,
+                                                _ptr(g_opd), n, _ptr(g_amp) if need_amp else None, n),
                        ctx, 'mftx_phase_grad_many')
         return g_opd, g_amp
 
