@@ -68,7 +68,7 @@ class BroadbandPSFModel:
     # ---------------------------------------------------------------- forward
     def forward_images(self, opd, amp):
         for d, div in enumerate(self.div):
-            self.plan.set_opd(0, opd + div, copy=d)
+            torch.add(opd, div, out=self.plan.opd_buffer(0, copy=d))
             self.plan.set_amplitude(0, amp, copy=d)
         self.plan(self._wide)
         return self.plan.images(self._wide).contiguous()

@@ -277,6 +277,10 @@ class PSFPlan:
         """Overwrite a tile's OPD (of one copy) in place (the plan re-reads it on every call)."""
         self._amp_opd[copy * self._ntile0 + tile][1].copy_(torch.as_tensor(opd, dtype=torch.float64))
 
+    def opd_buffer(self, tile, copy=0):
+        """The float64 device OPD a tile (of one copy) is read from, for in-place updates."""
+        return self._amp_opd[copy * self._ntile0 + tile][1]
+
     def set_amplitude(self, tile, amp, copy=0):
         """Overwrite a tile's (masked) amplitude (of one copy) in place."""
         self._amp_opd[copy * self._ntile0 + tile][0].copy_(torch.as_tensor(amp, dtype=torch.float32))
