@@ -276,6 +276,63 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
   }
   const float sc = P.scale;
   const bool has_p = jp < N, has_m = jm >= 0 && jm < N && jm != jp;
+  if (EPI == EPI_STORE_CPLX && P.trans) {
+    // Transposed complex store: C row r of output column j goes to
+    // dst[j * ldd + r].  As for the packed store, one lane swap per word
+    // gives lane l rows 0-7 and lane l+32 rows 8-15 of each 16-row block, so
+    // a lane stores 64 contiguous bytes as 16-byte pieces.
+    float2* dst = (float2*)P.dst;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float v[2][2][8];  // [side][group of the pair][re, im of rows e = 0..3]
+#pragma unroll
+      for (int gg = 0; gg < 2; ++gg) {
+        const int g = 2 * h + gg;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float pr = acc[0][4 * g + e], pi = acc[1][4 * g + e], qr = acc[2][4 * g + e], qi = acc[3][4 * g + e];
+          v[0][gg][2 * e] = (pr - qi) * sc;
+          v[0][gg][2 * e + 1] = (pi + qr) * sc;
+          v[1][gg][2 * e] = (pr + qi) * sc;
+          v[1][gg][2 * e + 1] = (pi - qr) * sc;
+        }
+      }
+      const int r0 = wk.tr * BM + wr * 32 + 16 * h + 8 * (lane >> 5);  // first of this lane's 8 rows
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        float o[16];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[side][0][q]),
+                                                          __float_as_uint(v[side][1][q]), false, false);
+          o[q] = __uint_as_float(r[0]);
+          o[8 + q] = __uint_as_float(r[1]);
+        }
+        if (!(side ? has_m : has_p)) continue;
+        const int j = side ? jm : jp;
+        float wv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const long long wi = (long long)j * P.ldw + r0 + t;
+          wv[t] = !P.wmap || r0 + t >= P.rows ? 1.0f
+                                              : (P.wdtype == 3 ? (float)((const double*)P.wmap)[wi]
+                                                               : ((const float*)P.wmap)[wi]);
+        }
+        float2* d = dst + (long long)j * P.ldd + r0;
+        if (r0 + 8 <= P.rows && (((uintptr_t)d) & 15) == 0) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            ((float4*)d)[t] = make_float4(o[4 * t] * wv[2 * t], o[4 * t + 1] * wv[2 * t],
+                                          o[4 * t + 2] * wv[2 * t + 1], o[4 * t + 3] * wv[2 * t + 1]);
+        } else {
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            if (r0 + t < P.rows) d[t] = make_float2(o[2 * t] * wv[t], o[2 * t + 1] * wv[t]);
+        }
+      }
+    }
+    return;
+  }
   if (EPI == EPI_STORE_PACKED_T && PREC == PREC_BF16X3) {
     // Lane l holds K 0-3 (group 2h) and 8-11 (group 2h+1) of a 16-K block of
     // its output row, lane l+32 K 4-7 and 12-15.  One lane swap per word
