@@ -164,6 +164,17 @@ typedef struct {
 } mftx_mft;
 int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision);
 
+/* Persistent form of mftx_dft2_many for a list that runs many times (no
+ * reference counterpart; iterative callers such as phase retrieval): the
+ * descriptors and the operand workspace are built once in plan-owned device
+ * memory, and each execute issues only the launches, on the context's
+ * current stream.  The entries' arrays must stay allocated; their contents
+ * may change between executes.  Plans on one context run in stream order. */
+typedef struct mftx_mft_plan mftx_mft_plan;
+int mftx_mft_plan_create(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision, mftx_mft_plan** out);
+int mftx_mft_plan_execute(mftx_mft_plan* plan);
+int mftx_mft_plan_destroy(mftx_mft_plan* plan);
+
 /* ---------------------------------------------------------------------------
  * Detector-grid post-processing of an accumulated image (device pointers,
  * real float32 (1) / float64 (3) data, row strides in elements).

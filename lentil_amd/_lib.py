@@ -72,6 +72,10 @@ SIGNATURES = {
                             c_i64, c_i64, c_int, c_dbl]),
     'mftx_multiply': (c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64]),
     'mftx_dft2_many': (c_int, [c_vp, ctypes.POINTER(MftxMft), ctypes.c_int32, ctypes.c_int32]),
+    'mftx_mft_plan_create': (c_int, [c_vp, ctypes.POINTER(MftxMft), ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.POINTER(c_vp)]),
+    'mftx_mft_plan_execute': (c_int, [c_vp]),
+    'mftx_mft_plan_destroy': (c_int, [c_vp]),
     'mftx_phase_grad_many': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, c_i64, c_vp,
                                      ctypes.c_int32, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64]),
     'mftx_zero': (c_int, [c_vp, c_vp, c_i64]),

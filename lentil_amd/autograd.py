@@ -15,8 +15,8 @@ fields (nothing is stored but the inputs) and applies the adjoint MFT:
     dL/d opd = sum (2 pi / l) (-2) Im(conj(D) f),  dL/d amp = sum 2 Re(conj(D) e^{i theta})
 
 The forward runs as one batched PSFPlan over every diversity; the backward is two
-grouped MFT calls (mftx_dft2_many: the weighted forward fields, then the
-adjoint with conjugate twiddles) and one mftx_phase_grad_many per diversity.
+persistent grouped MFT plans (mftx_mft_plan: the weighted forward fields, then
+the adjoint with conjugate twiddles) and one mftx_phase_grad_many per diversity.
 """
 import ctypes
 
@@ -108,8 +108,35 @@ class BroadbandPSFModel:
                 r.scale, r.conj = unit, 1
         st['fwd'], st['adj'] = fwd, adj
         st['wls'] = (ctypes.c_double * nw)(*[float(w) for w in self.wavelengths])
+        # persistent MFT plans: descriptors and operand workspace built once
+        lib = _lib.load()
+        ctx = _lib.context()
+        prec = _lib.precision_code(self.precision)
+        st['plans'] = []
+        for arr in (fwd, adj):
+            h = ctypes.c_void_p()
+            _lib.check(lib.mftx_mft_plan_create(ctx, arr, nd * nw, prec, ctypes.byref(h)), ctx,
+                       'mftx_mft_plan_create')
+            st['plans'].append(h.value)
         self._bw = st
         return st
+
+    def close(self):
+        """Release the device plans (also done when the model is collected)."""
+        st = getattr(self, '_bw', None)
+        if st:
+            for h in st['plans']:
+                _lib.load().mftx_mft_plan_destroy(h)
+            st['plans'] = []
+        plan = getattr(self, 'plan', None)
+        if plan is not None:
+            plan.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def backward_grads(self, opd, amp, grad_images, need_amp):
         """Gradients of sum_d <g_d, I_d> with respect to opd (and amp).
@@ -124,7 +151,6 @@ class BroadbandPSFModel:
         m, n = self.shape
         M = self.M
         nd, nw = len(self.div), len(self.wavelengths)
-        prec = _lib.precision_code(self.precision)
         st = self._backward_state(m, n, M)
         st['amp'].copy_(amp.detach())
         for d, div in enumerate(self.div):
@@ -132,8 +158,9 @@ class BroadbandPSFModel:
         st['gI'].copy_(grad_images.detach())
         g_opd = torch.zeros((m, n), dtype=torch.float64, device=self.dev)
         g_amp = torch.zeros((m, n), dtype=torch.float64, device=self.dev) if need_amp else None
-        _lib.check(lib.mftx_dft2_many(ctx, st['fwd'], nd * nw, prec), ctx, 'mftx_dft2_many (weighted forward)')
-        _lib.check(lib.mftx_dft2_many(ctx, st['adj'], nd * nw, prec), ctx, 'mftx_dft2_many (adjoint)')
+        fwd_plan, adj_plan = st['plans']
+        _lib.check(lib.mftx_mft_plan_execute(fwd_plan), ctx, 'mftx_mft_plan_execute (weighted forward)')
+        _lib.check(lib.mftx_mft_plan_execute(adj_plan), ctx, 'mftx_mft_plan_execute (adjoint)')
         for d in range(nd):
             _lib.check(lib.mftx_phase_grad_many(ctx, _ptr(st['amp']), _lib.DT_F64, n, _ptr(st['tots'][d]),
                                                 _lib.DT_F64, n, m, n, st['wls'], nw, _ptr(st['D'][d * nw]), n, m * n,

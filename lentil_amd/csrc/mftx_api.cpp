@@ -325,10 +325,29 @@ struct ManyGeo {
 };
 }  // namespace
 
-// All entries [0, count) in one pack launch per input kind and one GEMM launch
-// per stage.  A batch whose descriptors exceed one ring slot is split in two.
-static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int prec) {
-  std::vector<ManyGeo> geo(count);
+// The launches of a list of MFTs: all entries in one pack launch per input
+// kind and one GEMM launch per stage.  many_build derives the geometry and the
+// descriptors (workspace and descriptor addresses as offsets); many_place
+// rebases them onto a workspace and a device descriptor block and writes the
+// host copy of that block; many_launch issues the launches.
+struct ManyBatch {
+  int bn = 64;
+  size_t ws = 0, bytes = 0;  // workspace and descriptor block sizes
+  size_t o_wv = 0, o_ph = 0, o_pk = 0, o_p1 = 0, o_p2 = 0, o_cx = 0, o_w1 = 0, o_w2 = 0;
+  std::vector<ManyGeo> geo;
+  std::vector<PhasorWave> waves;
+  std::vector<PhasorOp> phops;
+  std::vector<PackOp> packs;
+  std::vector<int> wave_of;  // phasor wave of entry i, or -(pack index + 1)
+  std::vector<Problem> p1, p2;
+  std::vector<double> cx;
+  std::vector<Work> w1, w2;
+  long long ph_total = 0, pk_total = 0;
+};
+
+static void many_build(const mftx_mft* mfts, int32_t count, int prec, ManyBatch& mb) {
+  std::vector<ManyGeo>& geo = mb.geo;
+  geo.assign(count, ManyGeo{});
   int64_t wide1 = 0, wide2 = 0;
   for (int32_t i = 0; i < count; ++i) {
     const mftx_mft& q = mfts[i];
@@ -351,9 +370,10 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
     wide2 += (g.rows2 / BM) * ((g.g2.slots + 63) / 64);
   }
   // 128-column tiles when the larger stage still gives every CU a workgroup
-  const int bn = std::max(wide1, wide2) >= 256 ? 128 : 64;
+  const int bn = mb.bn = std::max(wide1, wide2) >= 256 ? 128 : 64;
   const int kg = k_granule(bn);
-  size_t ws = 0;
+  size_t& ws = mb.ws;
+  ws = 0;
   for (int32_t i = 0; i < count; ++i) {
     const mftx_mft& q = mfts[i];
     ManyGeo& g = geo[i];
@@ -368,11 +388,12 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
   }
 
   // ---- descriptors -------------------------------------------------------------
-  std::vector<PhasorWave> waves;
-  std::vector<PhasorOp> phops;
-  std::vector<PackOp> packs;
-  long long ph_total = 0, pk_total = 0;
-  std::vector<int> wave_of;  // phasor wave index of entry i (-1 for complex input)
+  std::vector<PhasorWave>& waves = mb.waves;
+  std::vector<PhasorOp>& phops = mb.phops;
+  std::vector<PackOp>& packs = mb.packs;
+  long long& ph_total = mb.ph_total;
+  long long& pk_total = mb.pk_total;
+  std::vector<int>& wave_of = mb.wave_of;
   for (int32_t i = 0; i < count; ++i) {
     const mftx_mft& q = mfts[i];
     const ManyGeo& g = geo[i];
@@ -437,8 +458,11 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
   if (!phops.empty())
     ph_total = phops.back().begin + phasor_threads(phops.back().rows_pad, phops.back().kpad, phops.back().nwave);
 
-  std::vector<Problem> p1(count), p2(count);
-  std::vector<double> cx;
+  std::vector<Problem>& p1 = mb.p1;
+  std::vector<Problem>& p2 = mb.p2;
+  p1.assign(count, Problem{});
+  p2.assign(count, Problem{});
+  std::vector<double>& cx = mb.cx;
   std::vector<std::vector<Work>> clus1, clus2;
   for (int32_t i = 0; i < count; ++i) {
     const mftx_mft& q = mfts[i];
@@ -489,61 +513,124 @@ static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
   };
   group_works(clus1, p1, true);
   group_works(clus2, p2, false);
-  const std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
+  mb.w1 = xcd_order(clus1);
+  mb.w2 = xcd_order(clus2);
 
   // one descriptor block: waves, phasor ops, pack ops, problems, chunk origins, works
   size_t off = 0;
   auto take = [&](size_t b) { const size_t o = off; off = round_up((int64_t)(off + b), 16); return o; };
-  const size_t o_wv = take(sizeof(PhasorWave) * waves.size());
-  const size_t o_ph = take(sizeof(PhasorOp) * phops.size());
-  const size_t o_pk = take(sizeof(PackOp) * packs.size());
-  const size_t o_p1 = take(sizeof(Problem) * p1.size());
-  const size_t o_p2 = take(sizeof(Problem) * p2.size());
-  const size_t o_cx = take(sizeof(double) * cx.size());
-  const size_t o_w1 = take(sizeof(Work) * w1.size());
-  const size_t o_w2 = take(sizeof(Work) * w2.size());
-  if (off > kSlotBytes) {
+  mb.o_wv = take(sizeof(PhasorWave) * waves.size());
+  mb.o_ph = take(sizeof(PhasorOp) * phops.size());
+  mb.o_pk = take(sizeof(PackOp) * packs.size());
+  mb.o_p1 = take(sizeof(Problem) * p1.size());
+  mb.o_p2 = take(sizeof(Problem) * p2.size());
+  mb.o_cx = take(sizeof(double) * cx.size());
+  mb.o_w1 = take(sizeof(Work) * mb.w1.size());
+  mb.o_w2 = take(sizeof(Work) * mb.w2.size());
+  mb.bytes = off;
+}
+
+// Rebase the batch's descriptors onto workspace `base` and the device
+// descriptor block `d`, and write the block's contents to host memory `h`.
+static void many_place(ManyBatch& mb, char* base, char* d, char* h) {
+  for (auto& op : mb.phops) {
+    op.waves = (const PhasorWave*)(d + mb.o_wv) + (uintptr_t)op.waves;
+    op.dst = base + (uintptr_t)op.dst;
+  }
+  for (size_t i = 0; i < mb.geo.size(); ++i) {
+    mb.p1[i].a = base + mb.geo[i].off1;
+    mb.p1[i].dst = base + mb.geo[i].off2;
+    mb.p2[i].a = base + mb.geo[i].off2;
+    if (mb.wave_of[i] < 0) mb.packs[-mb.wave_of[i] - 1].dst = base + mb.geo[i].off1;
+  }
+  std::memcpy(h + mb.o_wv, mb.waves.data(), sizeof(PhasorWave) * mb.waves.size());
+  std::memcpy(h + mb.o_ph, mb.phops.data(), sizeof(PhasorOp) * mb.phops.size());
+  std::memcpy(h + mb.o_pk, mb.packs.data(), sizeof(PackOp) * mb.packs.size());
+  std::memcpy(h + mb.o_p1, mb.p1.data(), sizeof(Problem) * mb.p1.size());
+  std::memcpy(h + mb.o_p2, mb.p2.data(), sizeof(Problem) * mb.p2.size());
+  std::memcpy(h + mb.o_cx, mb.cx.data(), sizeof(double) * mb.cx.size());
+  std::memcpy(h + mb.o_w1, mb.w1.data(), sizeof(Work) * mb.w1.size());
+  std::memcpy(h + mb.o_w2, mb.w2.data(), sizeof(Work) * mb.w2.size());
+}
+
+// The launches of a placed batch (descriptors at device address d).
+static hipError_t many_launch(const ManyBatch& mb, int prec, const char* d, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (!mb.phops.empty())
+    e = launch_phasor_group(prec, (const PhasorOp*)(d + mb.o_ph), (int)mb.phops.size(), mb.ph_total, s);
+  if (e == hipSuccess && !mb.packs.empty())
+    e = launch_pack_group(prec, (const PackOp*)(d + mb.o_pk), (int)mb.packs.size(), mb.pk_total, s);
+  if (e == hipSuccess)
+    e = launch_gemm(prec, EPI_STORE_PACKED_T, mb.bn, (const Work*)(d + mb.o_w1), (int)mb.w1.size(),
+                    (const Problem*)(d + mb.o_p1), (const double*)(d + mb.o_cx), s);
+  if (e == hipSuccess)
+    e = launch_gemm(prec, EPI_STORE_CPLX, mb.bn, (const Work*)(d + mb.o_w2), (int)mb.w2.size(),
+                    (const Problem*)(d + mb.o_p2), (const double*)(d + mb.o_cx), s);
+  return e;
+}
+
+// All entries [0, count) on the context stream with the context workspace and
+// one descriptor ring slot; a batch whose descriptors exceed a slot is split.
+static int dft2_many_run(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int prec) {
+  ManyBatch mb;
+  many_build(mfts, count, prec, mb);
+  if (mb.bytes > kSlotBytes) {
     if (count == 1) return fail(ctx, MFTX_EINVAL, "descriptor block too large (output too big for one call)");
     const int32_t h = count / 2;
     const int rc = dft2_many_run(ctx, mfts, h, prec);
     return rc ? rc : dft2_many_run(ctx, mfts + h, count - h, prec);
   }
-  int rc = ensure_ws(ctx, ws);
+  int rc = ensure_ws(ctx, mb.ws);
   if (rc) return rc;
-  char* base = (char*)ctx->ws;
   // the slot is reserved first: the descriptors hold device addresses inside it
   int slot;
   char *h, *d;
   rc = reserve_desc(ctx, &slot, &h, &d);
   if (rc) return rc;
-  for (auto& op : phops) {
-    op.waves = (const PhasorWave*)(d + o_wv) + (uintptr_t)op.waves;
-    op.dst = base + (uintptr_t)op.dst;
-  }
-  for (int32_t i = 0; i < count; ++i) {
-    p1[i].a = base + geo[i].off1;
-    p1[i].dst = base + geo[i].off2;
-    p2[i].a = base + geo[i].off2;
-    if (wave_of[i] < 0) packs[-wave_of[i] - 1].dst = base + geo[i].off1;
-  }
-  std::memcpy(h + o_wv, waves.data(), sizeof(PhasorWave) * waves.size());
-  std::memcpy(h + o_ph, phops.data(), sizeof(PhasorOp) * phops.size());
-  std::memcpy(h + o_pk, packs.data(), sizeof(PackOp) * packs.size());
-  std::memcpy(h + o_p1, p1.data(), sizeof(Problem) * p1.size());
-  std::memcpy(h + o_p2, p2.data(), sizeof(Problem) * p2.size());
-  std::memcpy(h + o_cx, cx.data(), sizeof(double) * cx.size());
-  std::memcpy(h + o_w1, w1.data(), sizeof(Work) * w1.size());
-  std::memcpy(h + o_w2, w2.data(), sizeof(Work) * w2.size());
-  rc = commit_desc(ctx, slot, off);
+  many_place(mb, (char*)ctx->ws, d, h);
+  rc = commit_desc(ctx, slot, mb.bytes);
   if (rc) return rc;
-  if (!phops.empty())
-    HIPCHK(ctx, launch_phasor_group(prec, (const PhasorOp*)(d + o_ph), (int)phops.size(), ph_total, ctx->stream));
-  if (!packs.empty())
-    HIPCHK(ctx, launch_pack_group(prec, (const PackOp*)(d + o_pk), (int)packs.size(), pk_total, ctx->stream));
-  HIPCHK(ctx, launch_gemm(prec, EPI_STORE_PACKED_T, bn, (const Work*)(d + o_w1), (int)w1.size(),
-                          (const Problem*)(d + o_p1), (const double*)(d + o_cx), ctx->stream));
-  HIPCHK(ctx, launch_gemm(prec, EPI_STORE_CPLX, bn, (const Work*)(d + o_w2), (int)w2.size(),
-                          (const Problem*)(d + o_p2), (const double*)(d + o_cx), ctx->stream));
+  HIPCHK(ctx, many_launch(mb, prec, d, ctx->stream));
+  return MFTX_OK;
+}
+
+// A persistent list of MFTs (mftx_mft_plan_*): plan-owned workspace and
+// descriptors, built once.
+struct mftx_mft_plan {
+  mftx_ctx* ctx = nullptr;
+  int prec = 0;
+  ManyBatch mb;
+  void* mem = nullptr;   // operand workspace, then the descriptor block
+  char* desc = nullptr;
+};
+
+// Argument checks shared by mftx_dft2_many and mftx_mft_plan_create.
+static int check_mfts(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision) {
+  if (count < 0 || (count > 0 && !mfts)) return fail(ctx, MFTX_EINVAL, "bad MFT list");
+  if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3) return fail(ctx, MFTX_EINVAL, "unknown precision");
+  for (int32_t i = 0; i < count; ++i) {
+    const mftx_mft& q = mfts[i];
+    const std::string at = "MFT " + std::to_string(i) + ": ";
+    if (!q.F) return fail(ctx, MFTX_EINVAL, at + "null output pointer");
+    if (q.m <= 0 || q.n <= 0 || q.M <= 0 || q.N <= 0) return fail(ctx, MFTX_EINVAL, at + "shapes must be positive");
+    if (q.m > (1 << 20) || q.n > (1 << 20) || q.M > (1 << 20) || q.N > (1 << 20))
+      return fail(ctx, MFTX_EINVAL, at + "dimension too large");
+    if (q.ldF < q.N) return fail(ctx, MFTX_EINVAL, at + "output row stride smaller than row length");
+    if (q.f) {
+      if (q.f_dtype < 0 || q.f_dtype > 3) return fail(ctx, MFTX_EINVAL, at + "unknown input dtype");
+      if (q.ldf < q.n) return fail(ctx, MFTX_EINVAL, at + "input row stride smaller than row length");
+    } else {
+      if (!q.amp || !q.opd) return fail(ctx, MFTX_EINVAL, at + "phasor input needs amp and opd");
+      if ((q.amp_dtype != 1 && q.amp_dtype != 3) || (q.opd_dtype != 1 && q.opd_dtype != 3))
+        return fail(ctx, MFTX_EINVAL, at + "amp/opd must be float32 or float64");
+      if (q.ld_amp < q.n || q.ld_opd < q.n) return fail(ctx, MFTX_EINVAL, at + "phasor row stride too small");
+      if (!(q.inv_wl > 0) || !std::isfinite(q.inv_wl)) return fail(ctx, MFTX_EINVAL, at + "inv_wl must be positive");
+    }
+    if (q.w) {
+      if (q.w_dtype != 1 && q.w_dtype != 3) return fail(ctx, MFTX_EINVAL, at + "weight must be float32 or float64");
+      if (q.ldw < q.N) return fail(ctx, MFTX_EINVAL, at + "weight row stride smaller than row length");
+    }
+  }
   return MFTX_OK;
 }
 
@@ -782,34 +869,57 @@ int mftx_phase_grad_many(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t 
 
 int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision) {
   if (!ctx) return MFTX_EHANDLE;
-  if (count < 0 || (count > 0 && !mfts)) return fail(ctx, MFTX_EINVAL, "bad MFT list");
-  if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3) return fail(ctx, MFTX_EINVAL, "unknown precision");
-  for (int32_t i = 0; i < count; ++i) {
-    const mftx_mft& q = mfts[i];
-    const std::string at = "MFT " + std::to_string(i) + ": ";
-    if (!q.F) return fail(ctx, MFTX_EINVAL, at + "null output pointer");
-    if (q.m <= 0 || q.n <= 0 || q.M <= 0 || q.N <= 0) return fail(ctx, MFTX_EINVAL, at + "shapes must be positive");
-    if (q.m > (1 << 20) || q.n > (1 << 20) || q.M > (1 << 20) || q.N > (1 << 20))
-      return fail(ctx, MFTX_EINVAL, at + "dimension too large");
-    if (q.ldF < q.N) return fail(ctx, MFTX_EINVAL, at + "output row stride smaller than row length");
-    if (q.f) {
-      if (q.f_dtype < 0 || q.f_dtype > 3) return fail(ctx, MFTX_EINVAL, at + "unknown input dtype");
-      if (q.ldf < q.n) return fail(ctx, MFTX_EINVAL, at + "input row stride smaller than row length");
-    } else {
-      if (!q.amp || !q.opd) return fail(ctx, MFTX_EINVAL, at + "phasor input needs amp and opd");
-      if ((q.amp_dtype != 1 && q.amp_dtype != 3) || (q.opd_dtype != 1 && q.opd_dtype != 3))
-        return fail(ctx, MFTX_EINVAL, at + "amp/opd must be float32 or float64");
-      if (q.ld_amp < q.n || q.ld_opd < q.n) return fail(ctx, MFTX_EINVAL, at + "phasor row stride too small");
-      if (!(q.inv_wl > 0) || !std::isfinite(q.inv_wl)) return fail(ctx, MFTX_EINVAL, at + "inv_wl must be positive");
-    }
-    if (q.w) {
-      if (q.w_dtype != 1 && q.w_dtype != 3) return fail(ctx, MFTX_EINVAL, at + "weight must be float32 or float64");
-      if (q.ldw < q.N) return fail(ctx, MFTX_EINVAL, at + "weight row stride smaller than row length");
-    }
-  }
-  if (count == 0) return MFTX_OK;
+  const int rc = check_mfts(ctx, mfts, count, precision);
+  if (rc) return rc;  if (count == 0) return MFTX_OK;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   return dft2_many_run(ctx, mfts, count, precision);
+}
+
+int mftx_mft_plan_create(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision,
+                         mftx_mft_plan** out) {
+  if (!ctx) return MFTX_EHANDLE;
+  if (!out) return fail(ctx, MFTX_EINVAL, "null argument");
+  *out = nullptr;
+  if (count <= 0) return fail(ctx, MFTX_EINVAL, "empty MFT list");
+  int rc = check_mfts(ctx, mfts, count, precision);
+  if (rc) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  mftx_mft_plan* plan = new mftx_mft_plan();
+  plan->ctx = ctx;
+  plan->prec = precision;
+  many_build(mfts, count, precision, plan->mb);
+  const size_t wsb = (size_t)round_up((int64_t)plan->mb.ws, 256);
+  if (hipMalloc(&plan->mem, wsb + plan->mb.bytes) != hipSuccess) {
+    delete plan;
+    return fail(ctx, MFTX_ENOMEM, "MFT plan allocation failed");
+  }
+  plan->desc = (char*)plan->mem + wsb;
+  std::vector<char> host(plan->mb.bytes);
+  many_place(plan->mb, (char*)plan->mem, plan->desc, host.data());
+  if (hipMemcpy(plan->desc, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(plan->mem);
+    delete plan;
+    return fail(ctx, MFTX_EDEVICE, "MFT plan upload failed");
+  }
+  *out = plan;
+  return MFTX_OK;
+}
+
+int mftx_mft_plan_execute(mftx_mft_plan* plan) {
+  if (!plan) return MFTX_EHANDLE;
+  mftx_ctx* ctx = plan->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, many_launch(plan->mb, plan->prec, plan->desc, ctx->stream));
+  return MFTX_OK;
+}
+
+int mftx_mft_plan_destroy(mftx_mft_plan* plan) {
+  if (!plan) return MFTX_EHANDLE;
+  (void)hipSetDevice(plan->ctx->device);
+  (void)hipStreamSynchronize(plan->ctx->stream);
+  if (plan->mem) (void)hipFree(plan->mem);
+  delete plan;
+  return MFTX_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -251,3 +251,39 @@ def test_phase_grad_many(L):
     ref_ga = ga0 + np.sum(2 * cd.real, axis=0)
     assert rel_l2(g.cpu().numpy() - g0, ref_g - g0) < 1e-6
     assert rel_l2(ga.cpu().numpy() - ga0, ref_ga - ga0) < 1e-6
+
+
+def test_mft_plan_repeats(L):
+    """A persistent MFT plan (mftx_mft_plan_*) over a mixed list equals the
+    oracle on every execute, with the inputs' contents changed in between."""
+    lib, ctx = L._lib.load(), L._lib.context()
+    rng = np.random.default_rng(14)
+    cases = [(40, 33, 50, 47, (0.011, 0.013), (0.5, -1.5), (1.0, 2.0)), (64, 64, 96, 96, (1 / 96, 1 / 96), (0, 0), (0, 0))]
+    fs, Fs, entries = [], [], []
+    for i, (m, n, M, N, alpha, shift, offset) in enumerate(cases):
+        fd = torch.empty((m, n), dtype=torch.complex128, device='cuda')
+        F = torch.empty((M, N), dtype=torch.complex64, device='cuda')
+        entries.append(_mft_entry(L, F, alpha, shift, offset, 1.0, i % 2, f=fd))
+        fs.append(fd)
+        Fs.append(F)
+    arr = (L._lib.MftxMft * len(entries))(*entries)
+    h = ctypes.c_void_p()
+    L._lib.check(lib.mftx_mft_plan_create(ctx, arr, len(entries), L._lib.precision_code('fp32'), ctypes.byref(h)),
+                 ctx, 'mftx_mft_plan_create')
+    try:
+        for rep in range(3):
+            hosts = []
+            for fd in fs:
+                x = _cplx(rng, *fd.shape)
+                fd.copy_(torch.from_numpy(x))
+                hosts.append(x)
+            L._lib.check(lib.mftx_mft_plan_execute(h), ctx, 'mftx_mft_plan_execute')
+            torch.cuda.synchronize()
+            for i, ((m, n, M, N, alpha, shift, offset), x, F) in enumerate(zip(cases, hosts, Fs)):
+                if i % 2:
+                    ref = np.conj(mft.dft2(np.conj(x), alpha, (M, N), shift, offset, unitary=False))
+                else:
+                    ref = mft.dft2(x, alpha, (M, N), shift, offset, unitary=False)
+                assert rel_l2(F.cpu().numpy(), ref) < TOL_FIELD['fp32'], (rep, i)
+    finally:
+        lib.mftx_mft_plan_destroy(h)
