@@ -113,21 +113,27 @@ class BroadbandPSFModel:
         ctx = _lib.context()
         prec = _lib.precision_code(self.precision)
         st['plans'] = []
-        for arr in (fwd, adj):
-            h = ctypes.c_void_p()
-            _lib.check(lib.mftx_mft_plan_create(ctx, arr, nd * nw, prec, ctypes.byref(h)), ctx,
-                       'mftx_mft_plan_create')
-            st['plans'].append(h.value)
+        try:
+            for arr in (fwd, adj):
+                h = ctypes.c_void_p()
+                _lib.check(lib.mftx_mft_plan_create(ctx, arr, nd * nw, prec, ctypes.byref(h)), ctx,
+                           'mftx_mft_plan_create')
+                st['plans'].append(h.value)
+        except Exception:
+            for h in st['plans']:
+                lib.mftx_mft_plan_destroy(h)
+            raise
         self._bw = st
         return st
 
     def close(self):
-        """Release the device plans (also done when the model is collected)."""
+        """Release the device plans (also done when the model is collected);
+        calling the model afterwards raises."""
         st = getattr(self, '_bw', None)
         if st:
             for h in st['plans']:
                 _lib.load().mftx_mft_plan_destroy(h)
-            st['plans'] = []
+            self._bw = None
         plan = getattr(self, 'plan', None)
         if plan is not None:
             plan.close()

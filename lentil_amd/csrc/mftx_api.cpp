@@ -16,6 +16,9 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -180,6 +183,20 @@ static int fail(mftx_ctx* ctx, int code, const std::string& msg) {
 
 static int hipfail(mftx_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, MFTX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Host-side exceptions (allocation failures of the plan builders' containers)
+// end at the C boundary as status codes.
+template <class F> static int guarded(mftx_ctx* ctx, F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(ctx, MFTX_ENOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ctx, MFTX_EDEVICE, e.what());
+  } catch (...) {
+    return fail(ctx, MFTX_EDEVICE, "unexpected host exception");
+  }
 }
 
 #define HIPCHK(ctx, expr)                                   \
@@ -602,6 +619,9 @@ struct mftx_mft_plan {
   ManyBatch mb;
   void* mem = nullptr;   // operand workspace, then the descriptor block
   char* desc = nullptr;
+  ~mftx_mft_plan() {
+    if (mem) (void)hipFree(mem);
+  }
 };
 
 // Argument checks shared by mftx_dft2_many and mftx_mft_plan_create.
@@ -639,6 +659,7 @@ extern "C" {
 int mftx_version(void) { return 10100; }
 
 int mftx_create(int device, mftx_ctx** out) {
+  return guarded(nullptr, [&]() -> int {
   if (!out) return MFTX_EINVAL;
   *out = nullptr;
   int n = 0;
@@ -660,6 +681,7 @@ int mftx_create(int device, mftx_ctx** out) {
   }
   *out = ctx;
   return MFTX_OK;
+  });
 }
 
 int mftx_destroy(mftx_ctx* ctx) {
@@ -868,15 +890,18 @@ int mftx_phase_grad_many(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t 
 }
 
 int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision) {
+  return guarded(ctx, [&]() -> int {
   if (!ctx) return MFTX_EHANDLE;
   const int rc = check_mfts(ctx, mfts, count, precision);
   if (rc) return rc;  if (count == 0) return MFTX_OK;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   return dft2_many_run(ctx, mfts, count, precision);
+  });
 }
 
 int mftx_mft_plan_create(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t precision,
                          mftx_mft_plan** out) {
+  return guarded(ctx, [&]() -> int {
   if (!ctx) return MFTX_EHANDLE;
   if (!out) return fail(ctx, MFTX_EINVAL, "null argument");
   *out = nullptr;
@@ -884,25 +909,23 @@ int mftx_mft_plan_create(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
   int rc = check_mfts(ctx, mfts, count, precision);
   if (rc) return rc;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  mftx_mft_plan* plan = new mftx_mft_plan();
+  std::unique_ptr<mftx_mft_plan> plan(new mftx_mft_plan());  // frees its device block on every exit
   plan->ctx = ctx;
   plan->prec = precision;
   many_build(mfts, count, precision, plan->mb);
   const size_t wsb = (size_t)round_up((int64_t)plan->mb.ws, 256);
   if (hipMalloc(&plan->mem, wsb + plan->mb.bytes) != hipSuccess) {
-    delete plan;
+    plan->mem = nullptr;
     return fail(ctx, MFTX_ENOMEM, "MFT plan allocation failed");
   }
   plan->desc = (char*)plan->mem + wsb;
   std::vector<char> host(plan->mb.bytes);
   many_place(plan->mb, (char*)plan->mem, plan->desc, host.data());
-  if (hipMemcpy(plan->desc, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(plan->mem);
-    delete plan;
+  if (hipMemcpy(plan->desc, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(ctx, MFTX_EDEVICE, "MFT plan upload failed");
-  }
-  *out = plan;
+  *out = plan.release();
   return MFTX_OK;
+  });
 }
 
 int mftx_mft_plan_execute(mftx_mft_plan* plan) {
@@ -917,7 +940,6 @@ int mftx_mft_plan_destroy(mftx_mft_plan* plan) {
   if (!plan) return MFTX_EHANDLE;
   (void)hipSetDevice(plan->ctx->device);
   (void)hipStreamSynchronize(plan->ctx->stream);
-  if (plan->mem) (void)hipFree(plan->mem);
   delete plan;
   return MFTX_OK;
 }
@@ -1094,6 +1116,7 @@ int mftx_remove_tilt(mftx_ctx* ctx, const void* opd, int opd_dtype, int64_t ld_o
 
 int mftx_zernike_basis(mftx_ctx* ctx, const double* rho, const double* theta, const void* mask, int mask_dtype,
                        int64_t npix, const int32_t* noll_nm, int32_t nmodes, int normalize, double* out) {
+  return guarded(ctx, [&]() -> int {
   if (!ctx) return MFTX_EHANDLE;
   if (!rho || !theta || !mask || !noll_nm || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
   if (mask_dtype != 1 && mask_dtype != 3) return fail(ctx, MFTX_EINVAL, "mask must be float32 or float64");
@@ -1146,6 +1169,7 @@ int mftx_zernike_basis(mftx_ctx* ctx, const double* rho, const double* theta, co
                              (const double*)(d + o_coef), (const int*)(d + o_pw), (const double*)(d + o_norm), out,
                              ctx->stream));
   return MFTX_OK;
+  });
 }
 
 int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes) {
@@ -1204,6 +1228,7 @@ extern "C" {
 int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const double* wavelengths, int32_t nwave,
                   const mftx_problem* problems, int32_t nprob, int64_t image_rows, int64_t image_cols,
                   int32_t precision, int32_t lambda_group, int32_t tile_n, mftx_plan** out) {
+  return guarded(ctx, [&]() -> int {
   if (!ctx) return MFTX_EHANDLE;
   if (!out || !tiles || !wavelengths || (!problems && nprob > 0)) return fail(ctx, MFTX_EINVAL, "null argument");
   *out = nullptr;
@@ -1396,7 +1421,9 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     for (const auto& r : group_runs(p1, rows, tile_n)) add_tiles(clus1, r.first, r.second, s1rows[r.first],
                                                                  p1[r.first].cols, tile_n);
   }
-  mftx_plan* plan = new mftx_plan();
+  // owned until returned: every failure or exception path frees the plan and its workspace
+  std::unique_ptr<mftx_plan, int (*)(mftx_plan*)> owned(new mftx_plan(), mftx_plan_destroy);
+  mftx_plan* plan = owned.get();
   plan->ctx = ctx;
   plan->prec = prec;
   plan->bn = tile_n;
@@ -1473,7 +1500,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     for (size_t g = 0; g < groups.size(); ++g) fb_off.push_back(lay.take(sizeof(float2) * image_rows * image_cols));
 
   if (hipMalloc(&plan->ws, lay.total) != hipSuccess) {
-    delete plan;
+    plan->ws = nullptr;
     return fail(ctx, MFTX_ENOMEM, "plan workspace allocation of " + std::to_string(lay.total) + " bytes failed");
   }
   char* base = (char*)plan->ws;
@@ -1512,8 +1539,6 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   up(plan->chunk_x, cx.data(), sizeof(double) * cx.size());
   up(plan->waves, iw.data(), sizeof(PhasorWave) * iw.size());
   if (e != hipSuccess) {
-    (void)hipFree(plan->ws);
-    delete plan;
     return hipfail(ctx, e, "plan descriptor upload");
   }
 
@@ -1547,8 +1572,6 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->phasor_threads = pthreads;
   e = hipMemcpy(plan->phasor_ops, ops.data(), sizeof(PhasorOp) * ops.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    (void)hipFree(plan->ws);
-    delete plan;
     return hipfail(ctx, e, "plan descriptor upload");
   }
   if (fast) {
@@ -1609,8 +1632,9 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->info.n_work1 = plan->nwork1;
   plan->info.n_work2 = plan->nwork2;
   plan->info.n_launch = (ops.empty() ? 0 : 1) + 2 + (int)plan->reduces.size();
-  *out = plan;
+  *out = owned.release();
   return MFTX_OK;
+  });
 }
 
 }  // extern "C"
@@ -1677,6 +1701,7 @@ static int plan_launch(mftx_plan* plan, void* image, int dtype, int64_t ldi, int
 extern "C" {
 
 int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi, int32_t accumulate) {
+  return guarded(plan ? plan->ctx : nullptr, [&]() -> int {
   if (!plan || !plan->ctx) return MFTX_EHANDLE;
   mftx_ctx* ctx = plan->ctx;
   if (!image) return fail(ctx, MFTX_EINVAL, "null image");
@@ -1714,6 +1739,7 @@ int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
   }
   HIPCHK(ctx, hipGraphLaunch(plan->gexec, ctx->stream));
   return MFTX_OK;
+  });
 }
 
 int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi, int32_t repeats, double* ms) {
