@@ -486,19 +486,20 @@ __device__ __forceinline__ void store_partial(const Work& wk, const Problem& P, 
   }
 }
 
-template <int PREC, int EPI, int WN, int KS>
-__global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void gemm_kernel(
-    const Work* __restrict__ works, const Problem* __restrict__ probs, const double* __restrict__ chunk_x) {
+// The work item's loop.  MF = false for a wave whose 32 rows all lie past
+// the problems' last row (the ragged last row tile of a small tile): it
+// stages its share of the operands and keeps the barriers but issues no
+// fragment reads or MFMAs, and its zero accumulators give the same zero rows
+// the padding would, so the wave sharing its SIMD runs alone.
+template <int PREC, int EPI, int WN, int KS, bool MF>
+__device__ __forceinline__ void gemm_body(const Work& wk, const Problem* __restrict__ probs,
+                                          const double* __restrict__ chunk_x, Lds<PREC, WN, KS>& lds) {
   using C = Cfg<WN, KS>;
   constexpr int BS = C::BS;
-  __shared__ Lds<PREC, WN, KS> lds;
-
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wid = tid >> 6;
   const int wr = wid / WN, wc = wid % WN;
-  const Work wk = works[blockIdx.x];
-  if (wk.pbeg >= wk.pend) return;
 
   // staging coordinates: A rows from a_row0, one slot / K group of twiddles
   const long long a_row0 = (long long)wk.tr * BM;
@@ -554,7 +555,9 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
       load_a<PREC, WN, KS>(sgp, a_row0, tid, ts, st);
       gen_b<PREC, WN, KS>(sx, sgp, q, st);
 
-      if (PREC == PREC_BF16X3) {
+      if (!MF) {
+        // no fragment reads or MFMAs for this wave
+      } else if (PREC == PREC_BF16X3) {
         auto& L = reinterpret_cast<Lds<PREC_BF16X3, WN, KS>&>(lds);
         const int r0 = wr * 32 + (lane & 31), c0 = wc * 32 + (lane & 31);
 #pragma unroll
@@ -607,6 +610,21 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
   }
   if (EPI == EPI_INTENSITY)
     store_partial(wk, probs[wk.pbeg], BS, wr, wc, lane, ip, im);
+}
+
+template <int PREC, int EPI, int WN, int KS>
+__global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void gemm_kernel(
+    const Work* __restrict__ works, const Problem* __restrict__ probs, const double* __restrict__ chunk_x) {
+  __shared__ Lds<PREC, WN, KS> lds;
+  const Work wk = works[blockIdx.x];
+  if (wk.pbeg >= wk.pend) return;
+  int rows = 0;
+  for (int p = wk.pbeg; p < wk.pend; ++p) rows = max(rows, probs[p].rows);
+  const int wr = (threadIdx.x >> 6) / WN;
+  if (wk.tr * BM + wr * 32 < rows)  // wave-uniform
+    gemm_body<PREC, EPI, WN, KS, true>(wk, probs, chunk_x, lds);
+  else
+    gemm_body<PREC, EPI, WN, KS, false>(wk, probs, chunk_x, lds);
 }
 
 template <int WN, int KS>
