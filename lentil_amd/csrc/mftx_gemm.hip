@@ -513,11 +513,13 @@ __device__ __forceinline__ void gemm_body(const Work& wk, const Problem* __restr
   ip = f32x16{};
   im = f32x16{};
 
-  // ---- prologue: chunk 0 into LDS buffer 0, chunk 1 staged in registers ----
+  // ---- prologue: chunk 0 into LDS buffer 0, chunks 1 and 2 staged in the two
+  // register sets (chunk k in set k & 1), so each global load has two chunks
+  // of work to arrive in ----
   int sp = wk.pbeg, ss = 0;  // staging cursor: next (problem, chunk) to stage
   StageProb sgp;
   float2 q[C::TW];
-  Staged<WN, KS> st;
+  Staged<WN, KS> st0, st1;
   set_stage<KS>(probs[sp], b_slot, sgp, q);
   double sx = twiddle_x<KS>(chunk_x, sgp, 0, st_g);
   auto advance = [&]() {
@@ -527,86 +529,93 @@ __device__ __forceinline__ void gemm_body(const Work& wk, const Problem* __restr
     }
     sx = twiddle_x<KS>(chunk_x, sgp, sp < wk.pend ? ss : 0, st_g);
   };
-  load_a<PREC, WN, KS>(sgp, a_row0, tid, 0, st);
-  gen_b<PREC, WN, KS>(sx, sgp, q, st);
-  write_b<PREC, WN, KS>(lds, 0, st_slot, st_g, st);
-  write_a<PREC, WN, KS>(lds, 0, tid, st);
-  advance();
-  {
-    // chunk 1 (or, past the last chunk, a valid chunk restaged, never read)
-    const int ts = sp < wk.pend ? ss : 0;
-    load_a<PREC, WN, KS>(sgp, a_row0, tid, ts, st);
-    gen_b<PREC, WN, KS>(sx, sgp, q, st);
-    if (sp < wk.pend) advance();
-  }
+  // stage the cursor's chunk (past the last chunk: a valid chunk restaged, never read)
+  auto stage = [&](Staged<WN, KS>& X) {
+    const bool more = sp < wk.pend;
+    load_a<PREC, WN, KS>(sgp, a_row0, tid, more ? ss : 0, X);
+    gen_b<PREC, WN, KS>(sx, sgp, q, X);
+    if (more) advance();
+  };
+  stage(st0);
+  write_b<PREC, WN, KS>(lds, 0, st_slot, st_g, st0);
+  write_a<PREC, WN, KS>(lds, 0, tid, st0);
+  stage(st1);
+  stage(st0);
   __syncthreads();
 
   int cur = 0;
-  for (int pc = wk.pbeg; pc < wk.pend; ++pc) {
-    const int nch = probs[pc].kpad / C::BKC;
-    for (int s = 0; s < nch; ++s) {
-      // The registers hold chunk s+1: store it into the idle buffer first (its
-      // LDS stores then overlap this chunk's MFMAs), then stage chunk s+2 into
-      // the registers (past the last chunk: a valid chunk restaged, never read).
-      write_b<PREC, WN, KS>(lds, cur ^ 1, st_slot, st_g, st);
-      write_a<PREC, WN, KS>(lds, cur ^ 1, tid, st);
-      const bool more = sp < wk.pend;
-      const int ts = more ? ss : 0;
-      load_a<PREC, WN, KS>(sgp, a_row0, tid, ts, st);
-      gen_b<PREC, WN, KS>(sx, sgp, q, st);
+  int pc = wk.pbeg, cs = 0, cn = probs[pc].kpad / C::BKC;  // compute cursor: problem, chunk, chunks
+  // Chunk s: store chunk s+1 from its register set X into the idle buffer (its
+  // LDS stores then overlap this chunk's MFMAs), stage chunk s+3 into X,
+  // compute chunk s, and close the problem after its last chunk.
+  auto chunk = [&](Staged<WN, KS>& X) {
+    write_b<PREC, WN, KS>(lds, cur ^ 1, st_slot, st_g, X);
+    write_a<PREC, WN, KS>(lds, cur ^ 1, tid, X);
+    stage(X);
 
-      if (!MF) {
-        // no fragment reads or MFMAs for this wave
-      } else if (PREC == PREC_BF16X3) {
-        auto& L = reinterpret_cast<Lds<PREC_BF16X3, WN, KS>&>(lds);
-        const int r0 = wr * 32 + (lane & 31), c0 = wc * 32 + (lane & 31);
+    if (!MF) {
+      // no fragment reads or MFMAs for this wave
+    } else if (PREC == PREC_BF16X3) {
+      auto& L = reinterpret_cast<Lds<PREC_BF16X3, WN, KS>&>(lds);
+      const int r0 = wr * 32 + (lane & 31), c0 = wc * 32 + (lane & 31);
 #pragma unroll
-        for (int j = 0; j < KS; ++j) {
-          // fragment rows are 16-aligned + (lane & 31): one swizzle key per lane
-          const int koff = swz<KS>(lane & 31, 2 * j + (lane >> 5));
-          bf16x8_t a[4], b[4];
+      for (int j = 0; j < KS; ++j) {
+        // fragment rows are 16-aligned + (lane & 31): one swizzle key per lane
+        const int koff = swz<KS>(lane & 31, 2 * j + (lane >> 5));
+        bf16x8_t a[4], b[4];
 #pragma unroll
-          for (int p = 0; p < 4; ++p) {
-            a[p] = *(const bf16x8_t*)&L.a[cur][p][r0][koff];
-            b[p] = *(const bf16x8_t*)&L.b[cur][p][c0][koff];
-          }
-          sblock_bf16(a, b, acc);
+        for (int p = 0; p < 4; ++p) {
+          a[p] = *(const bf16x8_t*)&L.a[cur][p][r0][koff];
+          b[p] = *(const bf16x8_t*)&L.b[cur][p][c0][koff];
         }
-        // schedule: the LDS stores of chunk s+1, the global loads of chunk s+2
-        // (consumed a whole chunk later), fragment reads, then the twiddle VALU
-        // work threaded between the MFMAs
-        SGB(SG_DSW, (KS == 1 ? 4 : 4) + 2 * C::A_ITEMS);
-        SGB(SG_VMEM_RD, 2 * C::A_ITEMS);
+        sblock_bf16(a, b, acc);
+      }
+      // schedule: the LDS stores of chunk s+1, the global loads of chunk s+3
+      // (consumed two chunks later), fragment reads, then the twiddle VALU
+      // work threaded between the MFMAs
+      SGB(SG_DSW, (KS == 1 ? 4 : 4) + 2 * C::A_ITEMS);
+      SGB(SG_VMEM_RD, 2 * C::A_ITEMS);
 #pragma unroll
-        for (int j = 0; j < KS; ++j) {
-          SGB(SG_DSR, 8);
+      for (int j = 0; j < KS; ++j) {
+        SGB(SG_DSR, 8);
 #pragma unroll
-          for (int i = 0; i < 12; ++i) {
-            SGB(SG_MFMA, 1);
-            SGB(SG_VALU, 3);
-          }
-        }
-      } else {
-        auto& L = reinterpret_cast<Lds<PREC_FP32, WN, KS>&>(lds);
-#pragma unroll
-        for (int ks = 0; ks < C::BKC / 2; ++ks) {
-          const int kk = 2 * ks + (lane >> 5);
-          const float2 a = L.a[cur][wr * 32 + (lane & 31)][kk];
-          const float2 b = L.b[cur][wc * 32 + (lane & 31)][kk];
-          acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc[0], 0, 0, 0);
-          acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.x, acc[1], 0, 0, 0);
-          acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acc[2], 0, 0, 0);
-          acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc[3], 0, 0, 0);
+        for (int i = 0; i < 12; ++i) {
+          SGB(SG_MFMA, 1);
+          SGB(SG_VALU, 3);
         }
       }
-
-      __syncthreads();
-      cur ^= 1;
-      if (more) advance();
-    }
-    problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im);
+    } else {
+      auto& L = reinterpret_cast<Lds<PREC_FP32, WN, KS>&>(lds);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
+      for (int ks = 0; ks < C::BKC / 2; ++ks) {
+        const int kk = 2 * ks + (lane >> 5);
+        const float2 a = L.a[cur][wr * 32 + (lane & 31)][kk];
+        const float2 b = L.b[cur][wc * 32 + (lane & 31)][kk];
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.x, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc[3], 0, 0, 0);
+      }
+    }
+
+    __syncthreads();
+    cur ^= 1;
+    if (++cs == cn) {
+      problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
+      if (++pc < wk.pend) {
+        cs = 0;
+        cn = probs[pc].kpad / C::BKC;
+      }
+    }
+  };
+  // two chunks per trip, so each register set has a fixed name
+  while (true) {
+    chunk(st1);
+    if (pc >= wk.pend) break;
+    chunk(st0);
+    if (pc >= wk.pend) break;
   }
   if (EPI == EPI_INTENSITY)
     store_partial(wk, probs[wk.pbeg], BS, wr, wc, lane, ip, im);
