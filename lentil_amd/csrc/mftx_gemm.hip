@@ -21,13 +21,14 @@
 //   WN = 1, KS = 1: 32 slots (64 output columns), 40 KiB LDS, two workgroups per CU;
 //   WN = 2, KS = 2: 64 slots (128 output columns), 32-wide K chunks, 96 KiB LDS,
 //                   one workgroup (8 waves) per CU.
-// One chunk ahead, each thread stages A items (32 bytes of one row: a BF16
-// plane or four complex64 values; global -> registers -> LDS) and generates
-// 2 KS consecutive twiddles of one slot, t_i = P * Q_i with
-// P = cis(alpha * x * Y) (FP64 argument and range reduction, FP32 sin/cos) per
-// chunk and the per-problem table Q_i = cis(alpha * i * Y).  The twiddle VALU
-// work is interleaved with the MFMAs of the current chunk
-// (sched_group_barrier), and the loop body is one basic block.
+// Two chunks ahead, in one of two register sets that alternate by chunk, each
+// thread stages A items (32 bytes of one row: a BF16 plane or four complex64
+// values; global -> registers -> LDS) and generates 2 KS consecutive twiddles
+// of one slot, t_i = P * Q_i with P = cis(alpha * x * Y) (FP64 argument and
+// range reduction, FP32 sin/cos) per chunk and the per-problem table
+// Q_i = cis(alpha * i * Y).  Scheduling hints (sched_group_barrier) ask for
+// the twiddle VALU work between the MFMAs; the compiler places most of it
+// after the chunk's barrier, next to the LDS stores that use it.
 //
 // Precision forms
 //   PREC_FP32   : complex64 A, v_mfma_f32_32x32x2_f32: 4 real MFMAs per
