@@ -209,8 +209,8 @@ def bench_retrieval(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 '
-                                   '(multi-GPU sharding is described in DESIGN.md, not built)'}))
+        print(json.dumps({'error': 'the path runs on one GPU; --gpus must be 1 '
+                                   '(sharding wavelengths or tiles across GPUs is not built)'}))
         sys.exit(2)
     import torch
     import lentil_amd as L
