@@ -173,3 +173,16 @@ def test_workloads_are_seeded():
     assert c2['amp'].shape == (512, 512) and len(c2['wavelengths']) == 4
     c3 = workloads.segmented_18(nwave=2)
     assert c3['mask'].shape == (18, 1024, 1024)
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py runs on one GPU: --gpus 2 exits with status 2 and says why,
+    before touching a device."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--gpus', '2'], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert 'one GPU' in msg['error']
