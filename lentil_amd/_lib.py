@@ -152,10 +152,13 @@ def context(dev=None):
     idx = dev.index if dev.index is not None else torch.cuda.current_device()
     ctx = _ctx.get(idx)
     if ctx is None:
-        h = c_vp()
-        check(lib.mftx_create(idx, ctypes.byref(h)), None, 'mftx_create')
-        ctx = h.value
-        _ctx[idx] = ctx
+        with _lock:  # one context per device, also when threads race here
+            ctx = _ctx.get(idx)
+            if ctx is None:
+                h = c_vp()
+                check(lib.mftx_create(idx, ctypes.byref(h)), None, 'mftx_create')
+                ctx = h.value
+                _ctx[idx] = ctx
     stream = torch.cuda.current_stream(idx).cuda_stream
     check(lib.mftx_set_stream(ctx, c_vp(stream)), ctx, 'mftx_set_stream')
     return ctx
