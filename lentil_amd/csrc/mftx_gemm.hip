@@ -26,9 +26,9 @@
 // values; global -> registers -> LDS) and generates 2 KS consecutive twiddles
 // of one slot, t_i = P * Q_i with P = cis(alpha * x * Y) (FP64 argument and
 // range reduction, FP32 sin/cos) per chunk and the per-problem table
-// Q_i = cis(alpha * i * Y).  Scheduling hints (sched_group_barrier) ask for
-// the twiddle VALU work between the MFMAs; the compiler places most of it
-// after the chunk's barrier, next to the LDS stores that use it.
+// Q_i = cis(alpha * i * Y).  The compiler schedules the loop (the twiddle VALU
+// work lands after the chunk's barrier, next to the LDS stores that use it;
+// scheduling hints and hand-threading it between the MFMAs measured no better).
 //
 // Precision forms
 //   PREC_FP32   : complex64 A, v_mfma_f32_32x32x2_f32: 4 real MFMAs per
@@ -233,8 +233,6 @@ __device__ __forceinline__ void write_a(Lds<PREC, WN, KS>& L, int buf, int tid, 
 }
 
 #define MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16((a), (b), (c), 0, 0, 0)
-#define SGB(mask, n) __builtin_amdgcn_sched_group_barrier((mask), (n), 0)
-constexpr int SG_VALU = 0x002, SG_MFMA = 0x008, SG_VMEM_RD = 0x020, SG_DSR = 0x100, SG_DSW = 0x200;
 
 // One 32x32 slot block, K = 16: P += A cos, Q += A sin for A = (re, im), each
 // real product as hi*hi + hi*lo + lo*hi.  a: planes (re_hi, re_lo, im_hi,
@@ -570,20 +568,6 @@ __device__ __forceinline__ void gemm_body(const Work& wk, const Problem* __restr
           b[p] = *(const bf16x8_t*)&L.b[cur][p][c0][koff];
         }
         sblock_bf16(a, b, acc);
-      }
-      // schedule: the LDS stores of chunk s+1, the global loads of chunk s+3
-      // (consumed two chunks later), fragment reads, then the twiddle VALU
-      // work threaded between the MFMAs
-      SGB(SG_DSW, (KS == 1 ? 4 : 4) + 2 * C::A_ITEMS);
-      SGB(SG_VMEM_RD, 2 * C::A_ITEMS);
-#pragma unroll
-      for (int j = 0; j < KS; ++j) {
-        SGB(SG_DSR, 8);
-#pragma unroll
-        for (int i = 0; i < 12; ++i) {
-          SGB(SG_MFMA, 1);
-          SGB(SG_VALU, 3);
-        }
       }
     } else {
       auto& L = reinterpret_cast<Lds<PREC_FP32, WN, KS>&>(lds);
