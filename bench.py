@@ -116,9 +116,11 @@ def bench_segmented(args, name):
     from oracle import optics
     cfg = getattr(workloads, name)()
     wl, wt = cfg['wavelengths'], cfg['weights']
+    tuned = args.lambda_group == 0 and args.tile_n == 0
     plan = L.PSFPlan(_segmented_pupil(L, cfg), wl, wt, pixelscale=cfg['pixelscale'], shape=cfg['shape'],
                      oversample=cfg['oversample'], tilts=[L.Tilt(*t) for t in cfg['tilts']],
-                     precision=args.precision, autotune=True)
+                     precision=args.precision, lambda_group=args.lambda_group or 8, tile_n=args.tile_n,
+                     autotune=tuned)
     out = torch.empty(plan.shape_out, dtype=torch.float32, device='cuda')
     for _ in range(args.warmup):
         plan(out)
@@ -148,7 +150,8 @@ def bench_segmented(args, name):
         'vs_baseline': None, 'dtype': args.precision, 'data': 'synthetic (seeded segmented pupil, Zernike OPD)',
         'config': {'workload': name, 'tiles': len(plan.tile_shapes), 'wavelengths': len(wl),
                    'mfts_per_step': plan.n_problems, 'detector': list(plan.shape_out),
-                   'lambda_group': plan.lambda_group, 'tile_n': plan.tile_n, 'parallelism': 'single GPU'},
+                   'lambda_group': plan.lambda_group, 'tile_n': plan.tile_n, 'autotuned': tuned,
+                   'parallelism': 'single GPU'},
         'mfts_per_s': plan.n_problems / dt, 'tflops_algorithmic': plan.info['flops'] / dt / 1e12,
         'stage_ms': ms,
         'roofline': {'bound': 'mfma', 'achieved': flops2 / (ms['stage2'] * 1e-3) / 1e12,
