@@ -48,7 +48,8 @@ typedef struct mftx_plan mftx_plan;
 int mftx_version(void);
 
 /* Context: owns a device, a stream binding and a growable workspace.
- * No counterpart in the reference (NumPy needs no context). */
+ * No counterpart in the reference (NumPy needs no context).  Plans keep a
+ * pointer to the context they were created on: destroy them first. */
 int mftx_create(int device, mftx_ctx** out);
 int mftx_destroy(mftx_ctx* ctx);
 const char* mftx_last_error(const mftx_ctx* ctx);
