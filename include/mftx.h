@@ -302,6 +302,18 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles,
  * row stride ldi.  accumulate != 0 adds to image (Wavefront.insert), else overwrites. */
 int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi,
                       int32_t accumulate);
+/* Complex fields from the plan's last execute (no reference counterpart;
+ * for gradients, which need the fields the image came from).  set_fields
+ * registers, once, per problem (the plan's problem order): the M x N complex64
+ * output F[u][v] (row stride ldd), an optional real weight map multiplying it
+ * (same layout, row stride ldw; wmap may be NULL, or hold NULL entries) and a
+ * scale.  A coherent group's field goes to its first problem's pointers.
+ * execute_fields then runs one complex-output stage-2 GEMM over the stage-1
+ * operands the last mftx_plan_execute left in the plan's workspace: call it
+ * after that execute and before the next. */
+int mftx_plan_set_fields(mftx_plan* plan, void* const* dst, int64_t ldd, const void* const* wmap, int32_t wmap_dtype,
+                         int64_t ldw, const double* scale);
+int mftx_plan_execute_fields(mftx_plan* plan);
 /* Capture the plan's launches in a hipGraph for this image pointer and replay it
  * on later executes with the same pointer (use_graph != 0), or run eagerly. */
 int mftx_plan_set_graph(mftx_plan* plan, int32_t use_graph);

@@ -96,6 +96,9 @@ SIGNATURES = {
                               ctypes.c_int32, ctypes.POINTER(MftxProblem), ctypes.c_int32, c_i64, c_i64,
                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(c_vp)]),
     'mftx_plan_execute': (c_int, [c_vp, c_vp, ctypes.c_int32, c_i64, ctypes.c_int32]),
+    'mftx_plan_set_fields': (c_int, [c_vp, ctypes.POINTER(c_vp), c_i64, ctypes.POINTER(c_vp), ctypes.c_int32, c_i64,
+                                     ctypes.POINTER(c_dbl)]),
+    'mftx_plan_execute_fields': (c_int, [c_vp]),
     'mftx_plan_set_graph': (c_int, [c_vp, ctypes.c_int32]),
     'mftx_plan_get_info': (c_int, [c_vp, ctypes.POINTER(MftxPlanInfo)]),
     'mftx_plan_profile': (c_int, [c_vp, c_vp, ctypes.c_int32, c_i64, ctypes.c_int32, ctypes.POINTER(c_dbl)]),

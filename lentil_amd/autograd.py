@@ -8,15 +8,18 @@ diversity OPD,
 
 which is the reference's broadband loop (docs/user/diffraction.rst:148-160,
 propagate.py:147-242 with a monolithic pupil).  Gradients flow through a
-``torch.autograd.Function`` whose backward pass recomputes the forward
-fields (nothing is stored but the inputs) and applies the adjoint MFT:
+``torch.autograd.Function`` whose backward pass forms the forward fields
+again from the forward plan's stage-1 operands (nothing else is stored) and
+applies the adjoint MFT:
 
     D = dL/d conj(f) = dft2_adjoint(w g F),   g = dL/dI
     dL/d opd = sum (2 pi / l) (-2) Im(conj(D) f),  dL/d amp = sum 2 Re(conj(D) e^{i theta})
 
-The forward runs as one batched PSFPlan over every diversity; the backward is two
-persistent grouped MFT plans (mftx_mft_plan: the weighted forward fields, then
-the adjoint with conjugate twiddles) and one mftx_phase_grad_many per diversity.
+The forward runs as one batched PSFPlan over every diversity.  The backward
+takes the weighted forward fields from that plan's own stage-1 operands
+(PSFPlan.fields: one complex-output GEMM), runs the adjoint MFTs as one
+persistent grouped plan (mftx_mft_plan, conjugate twiddles) and one
+mftx_phase_grad_many per diversity.
 """
 import ctypes
 
@@ -57,6 +60,7 @@ class BroadbandPSFModel:
                             oversample=oversample, precision=precision, lambda_group=lambda_group,
                             copies=len(self.div))
         self._wide = torch.empty(self.plan.shape_out, dtype=torch.float32, device=self.dev)
+        self._gen = 0  # forward calls so far: the plan's operands belong to the last one
         self.alphas = [(self.dx * self.du) / (wl * self.z * self.oversample) for wl in self.wavelengths]
 
     def __call__(self, opd, amp=None):
@@ -71,12 +75,14 @@ class BroadbandPSFModel:
             torch.add(opd, div, out=self.plan.opd_buffer(0, copy=d))
             self.plan.set_amplitude(0, amp, copy=d)
         self.plan(self._wide)
+        self._gen += 1
         return self.plan.images(self._wide).contiguous()
 
     # --------------------------------------------------------------- backward
     def _backward_state(self, m, n, M):
-        """Persistent device buffers of the backward pass and the two MFT lists
-        that point at them (built once: a call only refreshes the buffers)."""
+        """Persistent device buffers of the backward pass, the forward plan's
+        field outputs and the adjoint MFT plan that point at them (built once:
+        a call only refreshes the buffers)."""
         st = getattr(self, '_bw', None)
         if st is not None:
             return st
@@ -87,42 +93,33 @@ class BroadbandPSFModel:
                   gI=torch.empty((nd, M, M), dtype=torch.float32, device=dev),
                   Y=torch.empty((nd * nw, M, M), dtype=torch.complex64, device=dev),
                   D=torch.empty((nd * nw, m, n), dtype=torch.complex64, device=dev))
-        fwd = (_lib.MftxMft * (nd * nw))()
         adj = (_lib.MftxMft * (nd * nw))()
         for d in range(nd):
-            for j, (wl, wt, a) in enumerate(zip(self.wavelengths, self.weights, self.alphas)):
+            for j, a in enumerate(self.alphas):
                 i = d * nw + j
-                unit = float(abs(a))                      # sqrt(|a * a|), the unitary scale
-                q = fwd[i]
-                q.amp, q.amp_dtype, q.ld_amp = st['amp'].data_ptr(), _lib.DT_F64, n
-                q.opd, q.opd_dtype, q.ld_opd = st['tots'][d].data_ptr(), _lib.DT_F64, n
-                q.m, q.n, q.inv_wl = m, n, 1.0 / float(wl)
-                q.F, q.M, q.N, q.ldF = st['Y'][i].data_ptr(), M, M, M
-                q.alpha_r = q.alpha_c = a
-                q.scale = unit * float(wt)
-                q.w, q.w_dtype, q.ldw = st['gI'][d].data_ptr(), _lib.DT_F32, M
                 r = adj[i]
                 r.f, r.f_dtype, r.m, r.n, r.ldf = st['Y'][i].data_ptr(), _lib.DT_C64, M, M, M
                 r.F, r.M, r.N, r.ldF = st['D'][i].data_ptr(), m, n, n
                 r.alpha_r = r.alpha_c = a
-                r.scale, r.conj = unit, 1
-        st['fwd'], st['adj'] = fwd, adj
+                r.scale, r.conj = float(abs(a)), 1     # sqrt(|a * a|), the unitary scale
+        st['adj'] = adj
         st['wls'] = (ctypes.c_double * nw)(*[float(w) for w in self.wavelengths])
-        # persistent MFT plans: descriptors and operand workspace built once
+        # the weighted forward fields Y = unit w_l g_d F come out of the forward
+        # plan (problem order: diversity-major copies of one tile per wavelength)
+        Ys, Ws, scales = [], [], []
+        for p in self.plan.problems:
+            d, j = p['tile'] // self.plan._ntile0, p['wave']
+            Ys.append(st['Y'][d * nw + j])
+            Ws.append(st['gI'][d])
+            scales.append(float(abs(self.alphas[j])) * float(self.weights[j]))
+        self.plan.set_fields(Ys, scales, weights=Ws)
+        # the adjoint MFTs: one persistent plan (descriptors and workspace built once)
         lib = _lib.load()
         ctx = _lib.context()
         prec = _lib.precision_code(self.precision)
-        st['plans'] = []
-        try:
-            for arr in (fwd, adj):
-                h = ctypes.c_void_p()
-                _lib.check(lib.mftx_mft_plan_create(ctx, arr, nd * nw, prec, ctypes.byref(h)), ctx,
-                           'mftx_mft_plan_create')
-                st['plans'].append(h.value)
-        except Exception:
-            for h in st['plans']:
-                lib.mftx_mft_plan_destroy(h)
-            raise
+        h = ctypes.c_void_p()
+        _lib.check(lib.mftx_mft_plan_create(ctx, adj, nd * nw, prec, ctypes.byref(h)), ctx, 'mftx_mft_plan_create')
+        st['plans'] = [h.value]
         self._bw = st
         return st
 
@@ -144,14 +141,15 @@ class BroadbandPSFModel:
         except Exception:
             pass
 
-    def backward_grads(self, opd, amp, grad_images, need_amp):
+    def backward_grads(self, opd, amp, grad_images, need_amp, gen=None):
         """Gradients of sum_d <g_d, I_d> with respect to opd (and amp).
 
-        Two grouped MFT calls cover every (diversity, wavelength) pair: the
-        forward fields recomputed from the phasor with the weighted cotangent
-        w_l * g_d folded into the store, Y = w_l g_d F, then the adjoint MFT
-        D = dft2_adjoint(Y); one phase-gradient launch per diversity sums the
-        wavelengths."""
+        Two GEMM passes cover every (diversity, wavelength) pair: the forward
+        plan's fields from the stage-1 operands of the forward call ``gen``
+        (restaged from opd and amp when a later forward call replaced them)
+        with the weighted cotangent w_l * g_d folded into the store,
+        Y = w_l g_d F, then the adjoint MFT D = dft2_adjoint(Y); one
+        phase-gradient launch per diversity sums the wavelengths."""
         lib = _lib.load()
         ctx = _lib.context()
         m, n = self.shape
@@ -164,8 +162,10 @@ class BroadbandPSFModel:
         st['gI'].copy_(grad_images.detach())
         g_opd = torch.zeros((m, n), dtype=torch.float64, device=self.dev)
         g_amp = torch.zeros((m, n), dtype=torch.float64, device=self.dev) if need_amp else None
-        fwd_plan, adj_plan = st['plans']
-        _lib.check(lib.mftx_mft_plan_execute(fwd_plan), ctx, 'mftx_mft_plan_execute (weighted forward)')
+        if gen is None or gen != self._gen:
+            self.forward_images(opd.detach(), amp.detach())
+        self.plan.fields()
+        (adj_plan,) = st['plans']
         _lib.check(lib.mftx_mft_plan_execute(adj_plan), ctx, 'mftx_mft_plan_execute (adjoint)')
         for d in range(nd):
             _lib.check(lib.mftx_phase_grad_many(ctx, _ptr(st['amp']), _lib.DT_F64, n, _ptr(st['tots'][d]),
@@ -180,11 +180,13 @@ class _PSFStack(torch.autograd.Function):
     def forward(ctx, opd, amp, model):
         ctx.model = model
         ctx.save_for_backward(opd, amp)
-        return model.forward_images(opd.detach(), amp.detach())
+        images = model.forward_images(opd.detach(), amp.detach())
+        ctx.gen = model._gen
+        return images
 
     @staticmethod
     def backward(ctx, grad_images):
         opd, amp = ctx.saved_tensors
-        g_opd, g_amp = ctx.model.backward_grads(opd, amp, grad_images, ctx.needs_input_grad[1])
+        g_opd, g_amp = ctx.model.backward_grads(opd, amp, grad_images, ctx.needs_input_grad[1], gen=ctx.gen)
         return (g_opd.to(opd.dtype) if ctx.needs_input_grad[0] else None,
                 g_amp.to(amp.dtype) if g_amp is not None else None, None)

@@ -291,6 +291,42 @@ class PSFPlan:
         rows, cols = self.image_shape
         return out.view(rows, self.copies, cols).permute(1, 0, 2)
 
+    def set_fields(self, dst, scale, weights=None):
+        """Register, once, where ``fields()`` writes the complex field of each
+        problem (problem order): ``dst`` M x N complex64 device tensors with
+        one row stride, ``scale`` a number per problem, ``weights`` optional
+        real maps of the same layout multiplying the fields.  The tensors are
+        kept alive by the plan."""
+        n = self.n_problems
+        if len(dst) != n or len(scale) != n or (weights is not None and len(weights) != n):
+            raise ValueError(f'set_fields needs one entry per problem ({n})')
+        ld = dst[0].stride(0)
+        for t, p in zip(dst, self.problems):
+            if (t.dtype != torch.complex64 or t.device.type != 'cuda' or tuple(t.shape) != (p['M'], p['N'])
+                    or t.stride(1) != 1 or t.stride(0) != ld):
+                raise ValueError('fields must be M x N complex64 device tensors with one row stride')
+        wdt, ldw = _lib.DT_F32, 0
+        if weights is not None:
+            wdt = _lib.dtype_code(weights[0])
+            ldw = weights[0].stride(0)
+            for t, p in zip(weights, self.problems):
+                if (_lib.dtype_code(t) != wdt or wdt not in (_lib.DT_F32, _lib.DT_F64) or t.device.type != 'cuda'
+                        or tuple(t.shape) != (p['M'], p['N']) or t.stride(1) != 1 or t.stride(0) != ldw):
+                    raise ValueError('weight maps must be M x N real device tensors with one row stride')
+        self._fields_keep = (list(dst), None if weights is None else list(weights))
+        d = (ctypes.c_void_p * n)(*[t.data_ptr() for t in dst])
+        w = None if weights is None else (ctypes.c_void_p * n)(*[t.data_ptr() for t in weights])
+        sc = (ctypes.c_double * n)(*[float(x) for x in scale])
+        ctx = _lib.context()
+        _lib.check(_lib.load().mftx_plan_set_fields(self._plan, d, ld, w, wdt, ldw, sc), ctx, 'mftx_plan_set_fields')
+
+    def fields(self):
+        """Write each problem's field F[u][v] (times its weight map and scale)
+        into the tensors given to ``set_fields``, from the operands of this
+        plan's last call (one complex-output GEMM launch)."""
+        ctx = _lib.context()
+        _lib.check(_lib.load().mftx_plan_execute_fields(self._plan), ctx, 'mftx_plan_execute_fields')
+
     def close(self):
         if getattr(self, '_plan', None):
             _lib.load().mftx_plan_destroy(self._plan)

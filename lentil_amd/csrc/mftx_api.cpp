@@ -174,6 +174,14 @@ struct mftx_plan {
   void* g_image = nullptr;
   int g_dtype = -1, g_accum = -1;
   int64_t g_ld = -1;
+  // complex fields of every class from the stage-1 operands already in the
+  // workspace (mftx_plan_set_fields / mftx_plan_execute_fields)
+  Work* works_f = nullptr;
+  Problem* probs_f = nullptr;
+  int nwork_f = 0;
+  std::vector<Problem> pf;       // host copy, pointers filled by set_fields
+  std::vector<int> class_first;  // first member (problem index) of each class
+  int fields_ready = 0;
 };
 
 static int fail(mftx_ctx* ctx, int code, const std::string& msg) {
@@ -1478,6 +1486,17 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     }
   }
   std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
+  // the complex-field set: each class's stage-2 problem, one work set per class
+  std::vector<std::vector<Work>> clusf;
+  for (size_t c = 0; c < cls.size(); ++c) {
+    Problem q = p2[cls[c].p2];
+    q.dst = nullptr;
+    q.wmap = nullptr;
+    plan->pf.push_back(q);
+    plan->class_first.push_back(cls[c].members[0]);
+    add_tiles(clusf, (int)c, (int)c + 1, cls[c].rows2, q.cols, tile_n);
+  }
+  std::vector<Work> wf = xcd_order(clusf);
   // executed MFMA FLOPs of the two launches: per workgroup and problem, BM rows
   // x slots x kpad x 4 real products (re/im x cos/sin) x 2, times the passes
   const double passes = prec == PREC_BF16X3 ? 3.0 : 1.0;
@@ -1495,6 +1514,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   const size_t o_cx = lay.take(sizeof(double) * std::max<size_t>(cx.size(), 1));
   const size_t o_iw = lay.take(sizeof(PhasorWave) * std::max<size_t>(wave_slots.size(), 1));
   const size_t o_ops = lay.take(sizeof(PhasorOp) * std::max<size_t>(tops.size(), 1));
+  const size_t o_wf = lay.take(sizeof(Work) * std::max<size_t>(wf.size(), 1));
+  const size_t o_pf = lay.take(sizeof(Problem) * std::max<size_t>(plan->pf.size(), 1));
   std::vector<size_t> fb_off;
   if (!fast)
     for (size_t g = 0; g < groups.size(); ++g) fb_off.push_back(lay.take(sizeof(float2) * image_rows * image_cols));
@@ -1514,6 +1535,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   }
   for (auto& w : w2)
     if (w.partial) w.partial = (float*)(base + (size_t)(uintptr_t)w.partial);
+  for (size_t c = 0; c < cls.size(); ++c) plan->pf[c].a = base + patch2[cls[c].p2].a;
   std::vector<PhasorWave> iw(wave_slots.size());
   for (size_t i = 0; i < wave_slots.size(); ++i) {
     const auto& fx = stage1_fold.at(wave_slots[i]);
@@ -1528,6 +1550,9 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->waves = (PhasorWave*)(base + o_iw);
   plan->nwork1 = (int)w1.size();
   plan->nwork2 = (int)w2.size();
+  plan->works_f = (Work*)(base + o_wf);
+  plan->probs_f = (Problem*)(base + o_pf);
+  plan->nwork_f = (int)wf.size();
   hipError_t e = hipSuccess;
   auto up = [&](void* d, const void* s, size_t n) {
     if (e == hipSuccess && n) e = hipMemcpy(d, s, n, hipMemcpyHostToDevice);
@@ -1538,6 +1563,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   up(plan->probs2, p2_launch.data(), sizeof(Problem) * p2_launch.size());
   up(plan->chunk_x, cx.data(), sizeof(double) * cx.size());
   up(plan->waves, iw.data(), sizeof(PhasorWave) * iw.size());
+  up(plan->works_f, wf.data(), sizeof(Work) * wf.size());
   if (e != hipSuccess) {
     return hipfail(ctx, e, "plan descriptor upload");
   }
@@ -1777,6 +1803,45 @@ int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
   for (auto& x : ev) (void)hipEventDestroy(x);
   if (e != hipSuccess) return hipfail(ctx, e, "plan profile");
   for (int k = 0; k < 4; ++k) ms[k] = acc[k] / repeats;
+  return MFTX_OK;
+}
+
+int mftx_plan_set_fields(mftx_plan* plan, void* const* dst, int64_t ldd, const void* const* wmap, int32_t wmap_dtype,
+                         int64_t ldw, const double* scale) {
+  if (!plan || !plan->ctx) return MFTX_EHANDLE;
+  mftx_ctx* ctx = plan->ctx;
+  return guarded(ctx, [&]() -> int {
+    if (!dst || !scale) return fail(ctx, MFTX_EINVAL, "null argument");
+    if (wmap && wmap_dtype != 1 && wmap_dtype != 3) return fail(ctx, MFTX_EINVAL, "weight map must be float32 or float64");
+    for (size_t c = 0; c < plan->pf.size(); ++c) {
+      Problem& q = plan->pf[c];
+      const int i = plan->class_first[c];
+      const int64_t N = q.trans ? q.rows : q.ncols;  // the class's F is M x N, row stride ldd
+      if (!dst[i]) return fail(ctx, MFTX_EINVAL, "null field pointer for problem " + std::to_string(i));
+      if (ldd < N || (wmap && wmap[i] && ldw < N)) return fail(ctx, MFTX_EINVAL, "field row stride too small");
+      q.dst = dst[i];
+      q.ldd = ldd;
+      q.wmap = wmap ? wmap[i] : nullptr;
+      q.wdtype = wmap_dtype;
+      q.ldw = ldw;
+      q.scale = (float)scale[i];
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!plan->pf.empty())
+      HIPCHK(ctx, hipMemcpy(plan->probs_f, plan->pf.data(), sizeof(Problem) * plan->pf.size(), hipMemcpyHostToDevice));
+    plan->fields_ready = 1;
+    return MFTX_OK;
+  });
+}
+
+int mftx_plan_execute_fields(mftx_plan* plan) {
+  if (!plan || !plan->ctx) return MFTX_EHANDLE;
+  mftx_ctx* ctx = plan->ctx;
+  if (!plan->fields_ready) return fail(ctx, MFTX_EINVAL, "mftx_plan_set_fields has not been called");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, launch_gemm(plan->prec, EPI_STORE_CPLX, plan->bn, plan->works_f, plan->nwork_f, plan->probs_f,
+                          plan->chunk_x, ctx->stream));
   return MFTX_OK;
 }
 

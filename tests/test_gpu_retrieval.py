@@ -115,3 +115,32 @@ def test_phase_retrieval_512_gradient(L):
     err = rel_l2(x.grad.cpu().numpy(), g_ref)
     print(f'phase_retrieval_512 gradient rel-L2 vs complex128: {err:.2e}')
     assert err < 2e-4
+
+
+def test_gradient_after_another_forward(L):
+    """The backward reuses the forward plan's operands only while they belong
+    to its own forward call: with a second forward in between (other OPD), the
+    first graph's gradient is unchanged, and so is the second's."""
+    amp, opd, div, wl, wt, geom, target = _case(seed=51)
+    model = _model(L, amp, div, wl, wt, geom, 'bf16x3')
+    T = torch.tensor(target, device='cuda', dtype=torch.float64)
+    a = torch.tensor(amp, device='cuda')
+
+    def grad(o, interleave=None):
+        x = torch.tensor(o, device='cuda', requires_grad=True)
+        loss = torch.sum((model(x, a).double() - T) ** 2)
+        if interleave is not None:
+            y = torch.tensor(interleave, device='cuda', requires_grad=True)
+            loss2 = torch.sum((model(y, a).double() - T) ** 2)
+            loss.backward()
+            loss2.backward()
+            return x.grad.cpu().numpy(), y.grad.cpu().numpy()
+        loss.backward()
+        return x.grad.cpu().numpy()
+
+    opd2 = opd * 0.5
+    g1, g2 = grad(opd), grad(opd2)
+    h1, h2 = grad(opd, interleave=opd2)
+    assert rel_l2(h1, g1) < 1e-6
+    assert rel_l2(h2, g2) < 1e-6
+    model.close()
