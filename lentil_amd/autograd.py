@@ -71,9 +71,10 @@ class BroadbandPSFModel:
 
     # ---------------------------------------------------------------- forward
     def forward_images(self, opd, amp):
+        amp32 = amp.to(torch.float32)  # converted once for every copy
         for d, div in enumerate(self.div):
             torch.add(opd, div, out=self.plan.opd_buffer(0, copy=d))
-            self.plan.set_amplitude(0, amp, copy=d)
+            self.plan.set_amplitude(0, amp32, copy=d)
         self.plan(self._wide)
         self._gen += 1
         return self.plan.images(self._wide).contiguous()
