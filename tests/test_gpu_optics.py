@@ -361,3 +361,37 @@ def test_plan_copies(L):
         assert rel_l2(imgs[k].double().cpu().numpy(), ref) < 1e-6
         single.close()
     multi.close()
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_plan_fields_vs_oracle(L, precision):
+    """PSFPlan.fields: each problem's complex field (times a weight map and a
+    scale) from the operands of the plan's last call equals the complex128
+    dft2 of its phasor with the problem's geometry; calling it before
+    set_fields raises."""
+    from oracle import mft
+    n = 64
+    amp = inputs.normalize_power(inputs.circle((n, n), n // 2 - 3))
+    opd = inputs.zernike_compose(amp, [0, 0, 20e-9, -15e-9, 30e-9, 10e-9])
+    pupil = L.Pupil(amplitude=amp, opd=opd, pixelscale=1 / n, focal_length=10.0)
+    wl = np.array([550e-9, 650e-9, 750e-9])
+    plan = L.PSFPlan(pupil, wl, np.full(3, 1 / 3), pixelscale=5e-6, shape=16, oversample=2, precision=precision,
+                     lambda_group=2)
+    with pytest.raises(ValueError):
+        plan.fields()
+    rng = np.random.default_rng(33)
+    dst = [torch.empty((p['M'], p['N']), dtype=torch.complex64, device='cuda') for p in plan.problems]
+    wmaps = [torch.tensor(rng.uniform(0.5, 1.5, (p['M'], p['N'])), dtype=torch.float32, device='cuda')
+             for p in plan.problems]
+    scale = [0.5 + k for k in range(plan.n_problems)]
+    plan.set_fields(dst, scale, weights=wmaps)
+    plan()
+    plan.fields()
+    torch.cuda.synchronize()
+    for i, p in enumerate(plan.problems):
+        a_t, o_t = (t.cpu().numpy().astype(np.float64) for t in plan._amp_opd[p['tile']])
+        f = a_t * np.exp(2j * np.pi * o_t / plan.wavelengths[p['wave']])
+        ref = mft.dft2(f, p['alpha'], (p['M'], p['N']), p['shift'], p['offset'], unitary=False)
+        ref = ref * scale[i] * wmaps[i].double().cpu().numpy()
+        assert rel_l2(dst[i].cpu().numpy(), ref) < 1e-5, i
+    plan.close()
