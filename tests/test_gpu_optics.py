@@ -395,3 +395,36 @@ def test_plan_fields_vs_oracle(L, precision):
         ref = ref * scale[i] * wmaps[i].double().cpu().numpy()
         assert rel_l2(dst[i].cpu().numpy(), ref) < 1e-5, i
     plan.close()
+
+
+def test_plan_fields_coherent_groups(L):
+    """For a segmented pupil whose segments are summed coherently, a group's
+    field (the sum of its members' dft2 on the shared grid) goes to the
+    output of its first problem."""
+    from oracle import mft
+    p, wl, wt, _ = _small_segmented(L, seg_tilts=False)
+    plan = L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, tilts=[L.Tilt(0.3e-6, -0.2e-6)],
+                     precision='fp32', lambda_group=4)
+    assert plan.info['coherent_fast'] == 1
+    dst = [torch.zeros((q['M'], q['N']), dtype=torch.complex64, device='cuda') for q in plan.problems]
+    scale = [1.0 + 0.1 * k for k in range(plan.n_problems)]
+    plan.set_fields(dst, scale)
+    plan()
+    plan.fields()
+    torch.cuda.synchronize()
+    groups = {}
+    for i, q in enumerate(plan.problems):
+        groups.setdefault(q['group'], []).append(i)
+    assert len(groups) == len(wl)
+    for members in groups.values():
+        first = members[0]
+        q0 = plan.problems[first]
+        ref = np.zeros((q0['M'], q0['N']), dtype=complex)
+        for i in members:
+            q = plan.problems[i]
+            assert (q['M'], q['N'], q['out_r0'], q['out_c0']) == (q0['M'], q0['N'], q0['out_r0'], q0['out_c0'])
+            a_t, o_t = (t.cpu().numpy().astype(np.float64) for t in plan._amp_opd[q['tile']])
+            f = a_t * np.exp(2j * np.pi * o_t / plan.wavelengths[q['wave']])
+            ref += mft.dft2(f, q['alpha'], (q['M'], q['N']), q['shift'], q['offset'], unitary=False)
+        assert rel_l2(dst[first].cpu().numpy(), ref * scale[first]) < 1e-5
+    plan.close()
