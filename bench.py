@@ -46,6 +46,9 @@ def parse():
     ap.add_argument('--workload', default='broadband_512',
                     choices=['broadband_512', 'segmented_18', 'stress_2048', 'phase_retrieval_512'],
                     help='broadband_512 is the headline line; the others measure the remaining configurations')
+    ap.add_argument('--contraction', default='auto', choices=['auto', 'per_tile', 'merged'],
+                    help='segmented workloads: segment tiles one by one, summed into their union box, or '
+                         'whichever executes fewer MACs')
     ap.add_argument('--copy-mib', type=int, default=0,
                     help='after the timed steps, run one plain device copy of this many MiB (a known-size '
                          'kernel that scales hardware byte counters)')
@@ -117,25 +120,43 @@ def bench_segmented(args, name):
     cfg = getattr(workloads, name)()
     wl, wt = cfg['wavelengths'], cfg['weights']
     tuned = args.lambda_group == 0 and args.tile_n == 0
-    plan = L.PSFPlan(_segmented_pupil(L, cfg), wl, wt, pixelscale=cfg['pixelscale'], shape=cfg['shape'],
-                     oversample=cfg['oversample'], tilts=[L.Tilt(*t) for t in cfg['tilts']],
-                     precision=args.precision, lambda_group=args.lambda_group or 8, tile_n=args.tile_n,
-                     autotune=tuned)
+
+    def make(contraction, autotune):
+        return L.PSFPlan(_segmented_pupil(L, cfg), wl, wt, pixelscale=cfg['pixelscale'], shape=cfg['shape'],
+                         oversample=cfg['oversample'], tilts=[L.Tilt(*t) for t in cfg['tilts']],
+                         precision=args.precision, lambda_group=args.lambda_group or 8, tile_n=args.tile_n,
+                         autotune=autotune, contraction=contraction)
+
+    def timed(plan, out, steps, warmup):
+        for _ in range(warmup):
+            plan(out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan(out)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    plan = make(args.contraction, tuned)
     out = torch.empty(plan.shape_out, dtype=torch.float32, device='cuda')
-    for _ in range(args.warmup):
-        plan(out)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan(out)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    dt = timed(plan, out, args.steps, args.warmup)
     ms = plan.profile(torch.empty_like(out), repeats=3)
+    print(f'[bench] {name}: {1e3 * dt:.3f} ms/image, merged={plan.merged}', file=sys.stderr, flush=True)
+    per_tile = None
+    if plan.merged:
+        # the per-tile form of the same image: every (segment, wavelength) MFT in one grouped launch per stage
+        pt = make('per_tile', tuned)
+        dpt = timed(pt, out, max(2, args.steps // 4), 1)
+        per_tile = {'ms_per_step': 1e3 * dpt, 'psfs_per_s': len(wl) / dpt, 'mfts_per_step': pt.n_problems,
+                    'lambda_group': pt.lambda_group, 'tile_n': pt.tile_n,
+                    'stage_ms': pt.profile(torch.empty_like(out), repeats=2)}
+        pt.close()
     # error against the oracle on the two extreme wavelengths (bounded host work)
     idx = np.array([0, len(wl) - 1])
     sub = L.PSFPlan(_segmented_pupil(L, cfg), wl[idx], wt[idx], pixelscale=cfg['pixelscale'], shape=cfg['shape'],
                     oversample=cfg['oversample'], tilts=[L.Tilt(*t) for t in cfg['tilts']], precision=args.precision,
-                    lambda_group=plan.lambda_group, tile_n=plan.tile_n)
+                    lambda_group=plan.lambda_group, tile_n=plan.tile_n,
+                    contraction='merged' if plan.merged else 'per_tile')
     img = sub().double().cpu().numpy()
     p = optics.Pupil(cfg['amp'], cfg['opd'], cfg['dx'], cfg['focal_length'], mask=cfg['mask'])
     t1 = time.perf_counter()
@@ -148,10 +169,13 @@ def bench_segmented(args, name):
         'metric': f'PSFs/sec ({name})', 'value': len(wl) / dt, 'unit': 'PSFs/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': 1e3 * dt, 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': args.precision, 'data': 'synthetic (seeded segmented pupil, Zernike OPD)',
-        'config': {'workload': name, 'tiles': len(plan.tile_shapes), 'wavelengths': len(wl),
+        'config': {'workload': name, 'segments': int(cfg['mask'].shape[0]), 'wavelengths': len(wl),
+                   'contraction': 'merged' if plan.merged else 'per_tile', 'tiles': len(plan.tile_shapes),
                    'mfts_per_step': plan.n_problems, 'detector': list(plan.shape_out),
                    'lambda_group': plan.lambda_group, 'tile_n': plan.tile_n, 'autotuned': tuned,
                    'parallelism': 'single GPU'},
+        'contraction_cost_macs': plan.contraction_cost,
+        'per_tile': per_tile,
         'mfts_per_s': plan.n_problems / dt, 'tflops_algorithmic': plan.info['flops'] / dt / 1e12,
         'stage_ms': ms,
         'roofline': {'bound': 'mfma', 'achieved': flops2 / (ms['stage2'] * 1e-3) / 1e12,
@@ -284,20 +308,30 @@ def main():
         extra['autotune_ms_per_image'] = {f'lambda_group={lg},tile_n={tn}': round(t, 4)
                                           for (lg, tn), t in sorted(plan.tuning.items())}
     if not args.no_plain:
-        # plain per-wavelength FP32 path: the reference user loop through the drop-in API, device-resident image
+        # plain per-wavelength FP32 path: the reference user loop through the
+        # drop-in API with a frozen pupil (its amplitude and OPD tiles stay on
+        # the device, as the batched plan's do) and a device-resident image:
+        # per wavelength one phasor, one two-stage MFT and one insert
+        frozen = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'],
+                         focal_length=cfg['focal_length'])
+        frozen.freeze()
         img_dev = torch.zeros(plan.shape_out, dtype=torch.float64, device='cuda')
-        for rep in range(2):
+        for rep in range(3):
             img_dev.zero_()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for wl, wt in zip(cfg['wavelengths'], cfg['weights']):
-                w = L.Wavefront(wl) * pupil
+                w = L.Wavefront(wl) * frozen
                 w = L.propagate_dft(w, cfg['pixelscale'], shape=cfg['shape'], oversample=cfg['oversample'],
                                     precision='fp32')
                 w.insert(img_dev, weight=wt)
             torch.cuda.synchronize()
             dt_plain = time.perf_counter() - t0
-        extra['plain_fp32_per_wavelength'] = {'psfs_per_s': nwave / dt_plain, 'ms_per_image': 1e3 * dt_plain}
+        extra['plain_fp32_per_wavelength'] = {
+            'psfs_per_s': nwave / dt_plain, 'ms_per_image': 1e3 * dt_plain,
+            'tflops_algorithmic': total_flops / dt_plain / 1e12,
+            'frac_of_fp32_mfma_peak': total_flops / dt_plain / PEAK['fp32'],
+            'pupil': 'frozen (device-resident amplitude and OPD)'}
         extra['speedup_vs_plain_fp32'] = psfs_per_s / (nwave / dt_plain)
         plain_img = img_dev.cpu().numpy()
     if args.precision != 'fp32':

@@ -77,7 +77,8 @@ class BroadbandPSFModel:
             self.plan.set_amplitude(0, amp32, copy=d)
         self.plan(self._wide)
         self._gen += 1
-        return self.plan.images(self._wide).contiguous()
+        # a fresh tensor: the next forward call overwrites the plan's detector
+        return self.plan.images(self._wide).clone(memory_format=torch.contiguous_format)
 
     # --------------------------------------------------------------- backward
     def _backward_state(self, m, n, M):

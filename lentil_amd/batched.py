@@ -58,17 +58,106 @@ def coherent_groups(extents):
     return [g[0] for g in groups]
 
 
+def _round_up(v, q):
+    return -(-int(v) // q) * q
+
+
+def contraction_cost(shapes, M, N):
+    """Complex MACs the two GEMM stages execute for tiles of ``shapes``
+    (m, n) summed coherently onto one M x N grid, their stage-2 K
+    concatenated, with the kernels' padding (tiled rows to 128, K to 32),
+    for the cheaper of the two contraction orders."""
+    row_first = (sum(_round_up(m, 128) * _round_up(n, 32) * N for m, n in shapes)
+                 + _round_up(N, 128) * sum(_round_up(m, 32) for m, _ in shapes) * M)
+    col_first = (sum(_round_up(n, 128) * _round_up(m, 32) * M for m, n in shapes)
+                 + _round_up(M, 128) * sum(_round_up(n, 32) for _, n in shapes) * N)
+    return min(row_first, col_first)
+
+
+def _problems(tile_fields, wavelengths, weights, dx, du, z, prop_out, out_ext, oversample, shape_out):
+    problems = []
+    gid = 0
+    for w, (wl, wt) in enumerate(zip(wavelengths, weights)):
+        geo = []
+        for t, tf in enumerate(tile_fields):
+            g = field_geometry(tf, wl, dx, du, z, prop_out, out_ext, oversample)
+            if g is not None:
+                geo.append((t, g))
+        for members in coherent_groups([_ext.array_extent(g['shape'], g['out_offset']) for _, g in geo]):
+            for i in members:
+                t, g = geo[i]
+                M, N = (int(v) for v in g['shape'])
+                problems.append(dict(
+                    tile=t, wave=w, group=gid, M=M, N=N, alpha=tuple(float(a) for a in g['alpha']),
+                    shift=tuple(float(v) for v in g['shift']), offset=tuple(float(v) for v in g['offset']),
+                    out_offset=tuple(int(v) for v in g['out_offset']),
+                    out_r0=shape_out[0] // 2 - M // 2 + int(g['out_offset'][0]),
+                    out_c0=shape_out[1] // 2 - N // 2 + int(g['out_offset'][1]), weight=float(wt)))
+            gid += 1
+    return problems
+
+
+def _grid_key(p):
+    return p['M'], p['N'], p['alpha'], p['shift'], p['out_r0'], p['out_c0']
+
+
+def _mergeable(problems, ntiles, nwave):
+    """True when, at every wavelength, every tile lands in one coherent group
+    with one output grid (the same M, N, alpha, shift and placement): then
+    the group's coherent sum is one MFT of the tiles summed into their union
+    box (linearity; docs/user/segmented.rst:19-22)."""
+    by_wave = {}
+    for p in problems:
+        by_wave.setdefault(p['wave'], []).append(p)
+    if ntiles < 2 or len(by_wave) != nwave:
+        return False
+    for ps in by_wave.values():
+        if len(ps) != ntiles or len({p['group'] for p in ps}) != 1 or len({_grid_key(p) for p in ps}) != 1:
+            return False
+    return True
+
+
+def _union_tile(pupil, slices, tiles, tilt):
+    """The tiles summed into the bounding box of their slices of the pupil
+    grid: amplitude sum_t amp_t (each already times its segment mask, so
+    overlapping segments add as their coherent sum does), the pupil's OPD
+    over the box, and the box's offset from the pupil centre."""
+    r0 = min(s[0].start for s in slices)
+    r1 = max(s[0].stop for s in slices)
+    c0 = min(s[1].start for s in slices)
+    c1 = max(s[1].stop for s in slices)
+    amp = np.zeros((r1 - r0, c1 - c0))
+    for (a, _), s in zip(tiles, slices):
+        amp[s[0].start - r0:s[0].stop - r0, s[1].start - c0:s[1].stop - c0] += a
+    opd_all = np.asarray(pupil.opd)
+    if opd_all.size == 1:
+        opd = np.full(amp.shape, float(opd_all))
+    else:
+        opd = np.broadcast_to(opd_all, pupil.shape)[r0:r1, c0:c1]
+    box = np.s_[r0:r1, c0:c1]
+    return (amp, opd), _TileField(amp.shape, helper.slice_offset(box, pupil.shape), tilt)
+
+
 def plan_geometry(pupil, wavelengths, weights=None, pixelscale=None, shape=None, prop_shape=None, oversample=2,
-                  mask=None, tilts=()):
+                  mask=None, tilts=(), contraction='auto'):
     """Host bookkeeping of the reference loop for every (segment, wavelength).
 
     Returns dict(shape_out, wavelengths, weights, tiles=[(amp, opd)],
     problems=[dict(tile, wave, group, M, N, alpha, shift, offset, out_offset,
-    out_r0, out_c0, weight)]) where ``group`` numbers the coherent groups that
-    Wavefront.insert would merge before squaring (field.py:413-461) and
-    (out_r0, out_c0) is where F[0][0] lands on the detector
-    (field.py:267, before clipping).
+    out_r0, out_c0, weight)], merged, cost_per_tile, cost_merged) where
+    ``group`` numbers the coherent groups that Wavefront.insert would merge
+    before squaring (field.py:413-461) and (out_r0, out_c0) is where F[0][0]
+    lands on the detector (field.py:267, before clipping).
+
+    ``contraction`` chooses how the segments of a coherent group are
+    transformed: 'per_tile' runs one MFT per segment tile (their stage-2
+    GEMMs concatenated along K); 'merged' sums the tiles into their union
+    box first and runs one MFT per wavelength, which is exact when every
+    wavelength's tiles share one output grid (ValueError otherwise); 'auto'
+    merges when that is exact and executes fewer MACs.
     """
+    if contraction not in ('auto', 'per_tile', 'merged'):
+        raise ValueError("contraction must be 'auto', 'per_tile' or 'merged'")
     if not isinstance(pupil, Pupil):
         raise TypeError('PSFPlan needs a Pupil')
     if pupil.focal_length is None:
@@ -90,7 +179,7 @@ def plan_geometry(pupil, wavelengths, weights=None, pixelscale=None, shape=None,
     shape_out = tuple(int(v) for v in shape_out)
     prop_out = prop_shape * oversample
 
-    tiles, tile_fields = [], []
+    tiles, tile_fields, slices = [], [], []
     for n, s, amp, opd in pupil.phasor_tiles():
         amp = np.asarray(amp)
         opd = np.asarray(opd)
@@ -102,27 +191,31 @@ def plan_geometry(pupil, wavelengths, weights=None, pixelscale=None, shape=None,
         tiles.append((np.broadcast_to(amp, shp), np.broadcast_to(opd, shp)))
         tilt = ([pupil.tilt[n]] if pupil.tilt else []) + list(tilts)
         tile_fields.append(_TileField(shp, helper.slice_offset(s, pupil.shape), tilt))
+        slices.append(s)
 
-    problems = []
-    gid = 0
-    for w, (wl, wt) in enumerate(zip(wavelengths, weights)):
-        geo = []
-        for t, tf in enumerate(tile_fields):
-            g = field_geometry(tf, wl, dx, du, z, prop_out, out_ext, oversample)
-            if g is not None:
-                geo.append((t, g))
-        for members in coherent_groups([_ext.array_extent(g['shape'], g['out_offset']) for _, g in geo]):
-            for i in members:
-                t, g = geo[i]
-                M, N = (int(v) for v in g['shape'])
-                problems.append(dict(
-                    tile=t, wave=w, group=gid, M=M, N=N, alpha=tuple(float(a) for a in g['alpha']),
-                    shift=tuple(float(v) for v in g['shift']), offset=tuple(float(v) for v in g['offset']),
-                    out_offset=tuple(int(v) for v in g['out_offset']),
-                    out_r0=shape_out[0] // 2 - M // 2 + int(g['out_offset'][0]),
-                    out_c0=shape_out[1] // 2 - N // 2 + int(g['out_offset'][1]), weight=float(wt)))
-            gid += 1
-    return dict(shape_out=shape_out, wavelengths=wavelengths, weights=weights, tiles=tiles, problems=problems)
+    args = (wavelengths, weights, dx, du, z, prop_out, out_ext, oversample, shape_out)
+    problems = _problems(tile_fields, *args)
+    merged, cost_tiles, cost_merged = False, None, None
+    sliced = all(isinstance(s, tuple) and len(s) == 2 and all(isinstance(x, slice) for x in s) for s in slices)
+    if contraction != 'per_tile':
+        exact = sliced and _mergeable(problems, len(tiles), len(wavelengths))
+        if exact:
+            utile, ufield = _union_tile(pupil, slices, tiles, tile_fields[0].tilt)
+            uproblems = _problems([ufield], *args)
+            # the union box must land on the very grid the tiles did
+            exact = len(uproblems) == len(wavelengths) and all(
+                _grid_key(u) == _grid_key(next(p for p in problems if p['wave'] == u['wave'])) for u in uproblems)
+        if exact:
+            cost_tiles = sum(contraction_cost([tiles[p['tile']][0].shape for p in problems if p['wave'] == w],
+                                              *_grid_key(u)[:2]) for w, u in enumerate(uproblems))
+            cost_merged = sum(contraction_cost([utile[0].shape], *_grid_key(u)[:2]) for u in uproblems)
+            if contraction == 'merged' or cost_merged < cost_tiles:
+                tiles, problems, merged = [utile], uproblems, True
+        elif contraction == 'merged':
+            raise ValueError("contraction='merged' needs every wavelength's segment tiles to share one output "
+                             'grid (no per-segment tilts that split them)')
+    return dict(shape_out=shape_out, wavelengths=wavelengths, weights=weights, tiles=tiles, problems=problems,
+                merged=merged, cost_per_tile=cost_tiles, cost_merged=cost_merged)
 
 
 # (lambda_group, tile_n) chosen by PSFPlan(autotune=True), keyed by the
@@ -146,12 +239,20 @@ class PSFPlan:
     many independent copies of the pupil's tiles (each with its own device
     amplitude and OPD, see ``set_opd``) whose images lie side by side in one
     (rows, copies * cols) detector, all computed by the same launches.
+    ``contraction`` ('auto', 'per_tile' or 'merged', see ``plan_geometry``)
+    decides whether a segmented pupil's tiles are transformed one by one or
+    summed into their union box first; ``merged`` tells which was chosen,
+    and ``tile_shapes`` lists the tiles the plan transforms (the ones
+    ``set_opd`` and ``set_amplitude`` address).
     """
 
     def __init__(self, pupil, wavelengths, weights=None, pixelscale=None, shape=None, prop_shape=None,
                  oversample=2, mask=None, tilts=(), precision='bf16x3', lambda_group=8, tile_n=0,
-                 use_graph=True, autotune=False, copies=1):
-        geo = plan_geometry(pupil, wavelengths, weights, pixelscale, shape, prop_shape, oversample, mask, tilts)
+                 use_graph=True, autotune=False, copies=1, contraction='auto'):
+        geo = plan_geometry(pupil, wavelengths, weights, pixelscale, shape, prop_shape, oversample, mask, tilts,
+                            contraction)
+        self.merged = geo['merged']
+        self.contraction_cost = dict(per_tile=geo['cost_per_tile'], merged=geo['cost_merged'])
         copies = int(copies)
         if copies < 1:
             raise ValueError('copies must be positive')

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -162,7 +163,9 @@ struct mftx_plan {
     int64_t r0, c0;
   };
   struct Fallback {
-    float2* acc;
+    float2* acc;            // the plan's one accumulator, shared by the groups in launch order
+    int64_t br0, bc0;       // the group's union box on the image (clipped)
+    int64_t brows, bcols;
     std::vector<Add> adds;
     double weight;
   };
@@ -189,6 +192,9 @@ static int fail(mftx_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+int mftx::set_error(mftx_ctx* ctx, int code, const char* msg) { return fail(ctx, code, msg); }
+int mftx::context_device(const mftx_ctx* ctx) { return ctx->device; }
+
 static int hipfail(mftx_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, MFTX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -207,22 +213,36 @@ template <class F> static int guarded(mftx_ctx* ctx, F&& body) {
   }
 }
 
+// Device allocation whose failure is an ordinary status: the runtime's
+// pending error is cleared, so the next launch check (hipGetLastError after a
+// kernel launch) does not report this allocation's failure as its own.
+static bool try_malloc(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) == hipSuccess) return true;
+  *p = nullptr;
+  (void)hipGetLastError();
+  return false;
+}
+
 #define HIPCHK(ctx, expr)                                   \
   do {                                                      \
     hipError_t e_ = (expr);                                 \
     if (e_ != hipSuccess) return hipfail((ctx), e_, #expr); \
   } while (0)
 
+// Grow the context workspace to at least `bytes`, by at least half its old
+// size, so a run of slowly growing calls reallocates (and synchronises) only
+// a few times.
 static int ensure_ws(mftx_ctx* ctx, size_t bytes) {
   if (bytes <= ctx->ws_bytes) return MFTX_OK;
+  const size_t old = ctx->ws_bytes;
   if (ctx->ws) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx, hipFree(ctx->ws));
     ctx->ws = nullptr;
     ctx->ws_bytes = 0;
   }
-  const size_t nb = std::max(bytes, (size_t)(ctx->ws_bytes * 3 / 2));
-  if (hipMalloc(&ctx->ws, nb) != hipSuccess) {
+  const size_t nb = std::max(bytes, old + old / 2);
+  if (!try_malloc(&ctx->ws, nb)) {
     ctx->ws = nullptr;
     return fail(ctx, MFTX_ENOMEM, "workspace allocation of " + std::to_string(nb) + " bytes failed");
   }
@@ -261,7 +281,7 @@ static int ensure_aux(mftx_ctx* ctx, size_t bytes) {
     ctx->aux = nullptr;
     ctx->aux_bytes = 0;
   }
-  if (hipMalloc(&ctx->aux, bytes) != hipSuccess) {
+  if (!try_malloc(&ctx->aux, bytes)) {
     ctx->aux = nullptr;
     return fail(ctx, MFTX_ENOMEM, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
   }
@@ -677,7 +697,7 @@ int mftx_create(int device, mftx_ctx** out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess ||
       hipHostMalloc((void**)&ctx->ring_host, kRingSlots * kSlotBytes, hipHostMallocDefault) != hipSuccess ||
-      hipMalloc((void**)&ctx->ring_dev, kRingSlots * kSlotBytes) != hipSuccess) {
+      !try_malloc((void**)&ctx->ring_dev, kRingSlots * kSlotBytes)) {
     mftx_destroy(ctx);
     return MFTX_ENOMEM;
   }
@@ -687,6 +707,7 @@ int mftx_create(int device, mftx_ctx** out) {
       return MFTX_EDEVICE;
     }
   }
+
   *out = ctx;
   return MFTX_OK;
   });
@@ -724,6 +745,22 @@ int mftx_synchronize(mftx_ctx* ctx) {
 int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, int64_t ldf, void* F, int64_t M,
               int64_t N, int64_t ldF, double alpha_r, double alpha_c, double shift_r, double shift_c,
               double offset_r, double offset_c, double scale, int conj, int precision) {
+  return mftx::dft2_on_workspace(ctx, f, f_dtype, m, n, ldf, F, M, N, ldF, alpha_r, alpha_c, shift_r, shift_c,
+                                 offset_r, offset_c, scale, conj, precision, nullptr, 0);
+}
+
+}  // extern "C"
+
+// mftx_dft2 on the caller's workspace `ws` of `cap` bytes, or on the
+// context's growable workspace when ws is null.  The operands are fully
+// written before they are read (the packed input with its zero padding,
+// then stage 1's output for every K the second stage reads), so the
+// workspace needs no clearing; its padding rows beyond the output only feed
+// output rows that are never stored.
+int mftx::dft2_on_workspace(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, int64_t ldf, void* F,
+                            int64_t M, int64_t N, int64_t ldF, double alpha_r, double alpha_c, double shift_r,
+                            double shift_c, double offset_r, double offset_c, double scale, int conj, int precision,
+                            void* ws, size_t cap) {
   if (!ctx) return MFTX_EHANDLE;
   if (!f || !F) return fail(ctx, MFTX_EINVAL, "null array pointer");
   if (m <= 0 || n <= 0 || M <= 0 || N <= 0) return fail(ctx, MFTX_EINVAL, "shapes must be positive");
@@ -754,9 +791,17 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   const int kpad2 = (int)round_up(r1, kg);
   const size_t b1 = packed_bytes(prec, rows1, kpad1);
   const size_t b2 = packed_bytes(prec, rows2, kpad2);
-  int rc = ensure_ws(ctx, (size_t)round_up((int64_t)b1, 256) + b2);
-  if (rc) return rc;
-  char* a1 = (char*)ctx->ws;
+  const size_t need = (size_t)round_up((int64_t)b1, 256) + b2;
+  int rc = MFTX_OK;
+  if (!ws) {
+    rc = ensure_ws(ctx, need);
+    if (rc) return rc;
+    ws = ctx->ws;
+  } else if (need > cap) {
+    return fail(ctx, MFTX_EINVAL, "transform needs " + std::to_string(need) + " bytes of workspace, more than the " +
+                                      std::to_string(cap) + " available");
+  }
+  char* a1 = (char*)ws;
   char* a2 = a1 + round_up((int64_t)b1, 256);
   // twiddle coordinates: rows x = R + or against U - sr; columns x = S + oc
   // against V - sc; each stage's output shift is folded into its A operand
@@ -803,6 +848,8 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
   p2.trans = col_first ? 0 : 1;  // C is F (column-first) or F^T (row-first)
   return run_single(ctx, prec, EPI_STORE_CPLX, bn, p2, x02, rows2);
 }
+
+extern "C" {
 
 int mftx_phasor(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, const void* opd, int opd_dtype,
                 int64_t ld_opd, int64_t m, int64_t n, double wavelength, void* out) {
@@ -922,7 +969,7 @@ int mftx_mft_plan_create(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int
   plan->prec = precision;
   many_build(mfts, count, precision, plan->mb);
   const size_t wsb = (size_t)round_up((int64_t)plan->mb.ws, 256);
-  if (hipMalloc(&plan->mem, wsb + plan->mb.bytes) != hipSuccess) {
+  if (!try_malloc(&plan->mem, wsb + plan->mb.bytes)) {
     plan->mem = nullptr;
     return fail(ctx, MFTX_ENOMEM, "MFT plan allocation failed");
   }
@@ -1438,19 +1485,32 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->img_rows = image_rows;
   plan->img_cols = image_cols;
 
-  // stage-2 works: λ-groups of classes with one output geometry and orientation
-  // accumulate |F|^2 in registers (fast path); otherwise one work set per class
+  // stage-2 works: λ-groups of classes with one output window and orientation
+  // accumulate |F|^2 in registers (fast path); otherwise one work set per class.
+  // A work item maps every problem's slots through its first problem's output
+  // grid (store_partial), so classes share a λ-group only when their stage-2
+  // grids agree too: the pairing columns (cp, cm) and the slot count, which
+  // follow from the sub-pixel shift (DispersiveTilt moves it per wavelength
+  // while the integer placement stays).  Partial images are indexed by output
+  // pixel, so every λ-group of one window, whatever its grid, is summed by
+  // that window's single reduction.
   struct GeoInfo { std::vector<std::vector<int>> lgroups; size_t partial; int64_t prow, pcol; };
   std::vector<GeoInfo> geos;
   if (fast) {
-    std::map<std::tuple<int, int, int64_t, int64_t, int>, std::vector<int>> geo;
-    for (size_t c = 0; c < cls.size(); ++c)
-      geo[{cls[c].key.M, cls[c].key.N, cls[c].key.r0, cls[c].key.c0, cls[c].col_first ? 1 : 0}].push_back((int)c);
+    using Grid = std::tuple<int, int, int>;  // stage-2 (cp, cm, slots)
+    std::map<std::tuple<int, int, int64_t, int64_t, int>, std::map<Grid, std::vector<int>>> geo;
+    for (size_t c = 0; c < cls.size(); ++c) {
+      const Problem& q = p2[cls[c].p2];
+      geo[{cls[c].key.M, cls[c].key.N, cls[c].key.r0, cls[c].key.c0, cls[c].col_first ? 1 : 0}][{q.cp, q.cm, q.cols}]
+          .push_back((int)c);
+    }
     for (auto& kv : geo) {
       GeoInfo gi;
-      const auto& cl = kv.second;
-      for (size_t s = 0; s < cl.size(); s += lambda_group)
-        gi.lgroups.emplace_back(cl.begin() + s, cl.begin() + std::min(cl.size(), s + (size_t)lambda_group));
+      for (auto& sub : kv.second) {
+        const auto& cl = sub.second;
+        for (size_t s = 0; s < cl.size(); s += lambda_group)
+          gi.lgroups.emplace_back(cl.begin() + s, cl.begin() + std::min(cl.size(), s + (size_t)lambda_group));
+      }
       gi.prow = round_up(std::get<0>(kv.first), BM);  // partial[u][v]
       gi.pcol = round_up(std::get<1>(kv.first), BM);
       gi.partial = lay.take(sizeof(float) * gi.prow * gi.pcol * gi.lgroups.size());
@@ -1516,11 +1576,30 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   const size_t o_ops = lay.take(sizeof(PhasorOp) * std::max<size_t>(tops.size(), 1));
   const size_t o_wf = lay.take(sizeof(Work) * std::max<size_t>(wf.size(), 1));
   const size_t o_pf = lay.take(sizeof(Problem) * std::max<size_t>(plan->pf.size(), 1));
-  std::vector<size_t> fb_off;
-  if (!fast)
-    for (size_t g = 0; g < groups.size(); ++g) fb_off.push_back(lay.take(sizeof(float2) * image_rows * image_cols));
+  // partial-overlap groups: each group's union box on the image (its Fields'
+  // placements clipped to the image, field.py:282-296); one accumulator of
+  // the largest box serves every group in turn (same stream, in order)
+  std::vector<std::array<int64_t, 4>> fb_box;  // r0, c0, rows, cols per group (groups' map order)
+  size_t fb_acc = 0;
+  if (!fast) {
+    int64_t most = 0;
+    for (auto& kv : groups) {
+      int64_t r0 = image_rows, c0 = image_cols, r1 = 0, c1 = 0;
+      for (int c : kv.second) {
+        const ClassKey& K = cls[c].key;
+        r0 = std::min(r0, std::max<int64_t>(K.r0, 0));
+        c0 = std::min(c0, std::max<int64_t>(K.c0, 0));
+        r1 = std::max(r1, std::min<int64_t>(K.r0 + K.M, image_rows));
+        c1 = std::max(c1, std::min<int64_t>(K.c0 + K.N, image_cols));
+      }
+      const int64_t rows = std::max<int64_t>(r1 - r0, 0), cols = std::max<int64_t>(c1 - c0, 0);
+      fb_box.push_back({r0, c0, rows, cols});
+      most = std::max(most, rows * cols);
+    }
+    fb_acc = lay.take(sizeof(float2) * (size_t)std::max<int64_t>(most, 1));
+  }
 
-  if (hipMalloc(&plan->ws, lay.total) != hipSuccess) {
+  if (!try_malloc(&plan->ws, lay.total)) {
     plan->ws = nullptr;
     return fail(ctx, MFTX_ENOMEM, "plan workspace allocation of " + std::to_string(lay.total) + " bytes failed");
   }
@@ -1611,7 +1690,12 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     size_t g = 0;
     for (auto& kv : groups) {
       mftx_plan::Fallback fb;
-      fb.acc = (float2*)(base + fb_off[g++]);
+      fb.acc = (float2*)(base + fb_acc);
+      fb.br0 = fb_box[g][0];
+      fb.bc0 = fb_box[g][1];
+      fb.brows = fb_box[g][2];
+      fb.bcols = fb_box[g][3];
+      ++g;
       fb.weight = problems[cls[kv.second[0]].members[0]].weight;
       for (int c : kv.second) {
         const ClassKey& K = cls[c].key;
@@ -1676,7 +1760,7 @@ static hipError_t launch_reduces(const mftx_plan* plan, void* image, int dtype, 
   for (auto& r : plan->reduces) {
     int acc = accumulate;
     if (!accumulate && first && !(r.r0 == 0 && r.c0 == 0 && r.rows == plan->img_rows && r.cols == plan->img_cols)) {
-      hipError_t e = launch_zero(image, esz * plan->img_rows * ldi, s);
+      hipError_t e = launch_zero_rows(image, (int)esz, plan->img_rows, plan->img_cols, ldi, s);
       if (e != hipSuccess) return e;
       acc = 1;
     } else if (!accumulate && !first) {
@@ -1687,7 +1771,7 @@ static hipError_t launch_reduces(const mftx_plan* plan, void* image, int dtype, 
                                           (int)plan->img_rows, (int)plan->img_cols, (int)r.r0, (int)r.c0, acc, s);
     if (e != hipSuccess) return e;
   }
-  if (!accumulate && plan->reduces.empty()) return launch_zero(image, esz * plan->img_rows * ldi, s);
+  if (!accumulate && plan->reduces.empty()) return launch_zero_rows(image, (int)esz, plan->img_rows, plan->img_cols, ldi, s);
   return hipSuccess;
 }
 
@@ -1695,20 +1779,21 @@ static hipError_t launch_fallbacks(const mftx_plan* plan, void* image, int dtype
                                    hipStream_t s) {
   const size_t esz = dtype == 3 ? 8 : 4;
   hipError_t e = hipSuccess;
-  if (!accumulate) e = launch_zero(image, esz * plan->img_rows * ldi, s);
+  if (!accumulate) e = launch_zero_rows(image, (int)esz, plan->img_rows, plan->img_cols, ldi, s);
   for (auto& fb : plan->fallbacks) {
-    if (e == hipSuccess) e = launch_zero(fb.acc, sizeof(float2) * plan->img_rows * plan->img_cols, s);
+    if (fb.brows <= 0 || fb.bcols <= 0) continue;  // every Field of the group lies off the image
+    if (e == hipSuccess) e = launch_zero(fb.acc, sizeof(float2) * fb.brows * fb.bcols, s);
     for (auto& a : fb.adds) {
-      // clip F's placement to the image (field.py:282-296)
+      // clip F's placement to the image (field.py:282-296); acc holds the group's union box
       const int64_t or0 = std::max<int64_t>(a.r0, 0), oc0 = std::max<int64_t>(a.c0, 0);
       const int64_t or1 = std::min<int64_t>(a.r0 + a.M, plan->img_rows), oc1 = std::min<int64_t>(a.c0 + a.N, plan->img_cols);
       if (or1 <= or0 || oc1 <= oc0 || e != hipSuccess) continue;
       e = launch_insert(a.F, a.N, (int)(or0 - a.r0), (int)(oc0 - a.c0), (int)(or1 - or0), (int)(oc1 - oc0), fb.acc, 0,
-                        plan->img_cols, (int)or0, (int)oc0, 0, 1.0, s);
+                        fb.bcols, (int)(or0 - fb.br0), (int)(oc0 - fb.bc0), 0, 1.0, s);
     }
     if (e == hipSuccess)
-      e = launch_insert(fb.acc, plan->img_cols, 0, 0, (int)plan->img_rows, (int)plan->img_cols, image, dtype, ldi, 0, 0,
-                        1, fb.weight, s);
+      e = launch_insert(fb.acc, fb.bcols, 0, 0, (int)fb.brows, (int)fb.bcols, image, dtype, ldi, (int)fb.br0,
+                        (int)fb.bc0, 1, fb.weight, s);
   }
   return e;
 }

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct mftx_ctx;
+
 namespace mftx {
 
 // Tile geometry of the complex GEMM: WAVES_M x WN waves per workgroup, each
@@ -164,6 +166,8 @@ hipError_t launch_insert(const void* field, long long ldf, int fr0, int fc0, int
 hipError_t launch_multiply(const void* a, long long lda, const void* b, long long ldb, void* out,
                            long long ldo, int h, int w, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+// rows x cols elements of esz (4 or 8) bytes at row stride ld elements
+hipError_t launch_zero_rows(void* p, int esz, long long rows, long long cols, long long ld, hipStream_t s);
 hipError_t launch_weight_field(const void* F, long long ldF, const void* w, int w_dtype, long long ldw,
                                double scale, void* out, long long ldo, int h, int wd, hipStream_t s);
 // nwave wavelengths (inv_wls on the device, or inv_wl0 alone when null), D_w at D + w * strideD
@@ -205,5 +209,17 @@ hipError_t launch_remove_tilt(const void* opd, int odt, long long ld_opd, const 
 hipError_t launch_zernike(const double* rho, const double* theta, const void* mask, int mdt, long long npix,
                           int nmodes, const int* desc, const double* coef, const int* pw, const double* norm,
                           double* out, hipStream_t s);
+
+// host side of the C ABI (mftx_api.cpp)
+// mftx_dft2 on the caller's workspace ws (cap bytes), or on the context's own
+// growable workspace when ws is null
+int dft2_on_workspace(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, int64_t ldf, void* F,
+                      int64_t M, int64_t N, int64_t ldF, double alpha_r, double alpha_c, double shift_r,
+                      double shift_c, double offset_r, double offset_c, double scale, int conj, int precision,
+                      void* ws, size_t cap);
+// record msg as the context's last error and return code
+int set_error(mftx_ctx* ctx, int code, const char* msg);
+// the device a context was created on
+int context_device(const mftx_ctx* ctx);
 
 }  // namespace mftx

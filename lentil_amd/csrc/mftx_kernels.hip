@@ -417,6 +417,27 @@ __global__ void zero_kernel(uint4* __restrict__ p, long long n16, unsigned char*
   if (t < ntail) tail[t] = 0;
 }
 
+// Zero rows x cols elements of esz bytes (4 or 8) at row stride ld elements;
+// the bytes between rows and after the last row are left alone.
+__global__ void zero_rows_kernel(void* __restrict__ p, int esz, long long rows, long long cols, long long ld) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * cols) return;
+  const long long o = (t / cols) * ld + t % cols;
+  if (esz == 8) ((unsigned long long*)p)[o] = 0ull;
+  else ((unsigned int*)p)[o] = 0u;
+}
+
+hipError_t launch_zero_rows(void* p, int esz, long long rows, long long cols, long long ld, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  if (esz != 4 && esz != 8) return hipErrorInvalidValue;
+  if (ld == cols && ((uintptr_t)p & 15) == 0) return launch_zero(p, (size_t)esz * rows * cols, s);
+  const long long total = rows * cols;
+  const int bs = 256;
+  hipLaunchKernelGGL(zero_rows_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, p, esz, rows, cols,
+                     ld);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
   if (bytes == 0) return hipSuccess;
   const long long n16 = (long long)(bytes / 16);

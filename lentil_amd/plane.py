@@ -174,21 +174,40 @@ class Plane(_PlaneBase):
         if ptype is not None:
             self._ptype = as_ptype(ptype)
 
+    def _frozen_cache(self, mask):
+        """The frozen cache if it was built from ``mask``, else None.  While a
+        plane is frozen its amplitude and OPD properties return the stored
+        values (plane.py:73-108), but the reference reads the mask on every
+        multiply (plane.py:518-520), so the cache is used only for the mask it
+        was built from (the same array object, or an equal one: an in-place
+        edit of the mask array a frozen plane already holds is not seen until
+        ``thaw``)."""
+        c = self._cache
+        if not self.frozen or c is None:
+            return None
+        m = c['mask']
+        if mask is m or (np.shape(mask) == np.shape(m) and np.array_equal(mask, m)):
+            return c
+        return None
+
     @property
     def _slice(self):
-        if self.frozen and self._cache is not None:
-            return self._cache['slices']
-        return _plane_slice(self.mask)
+        mask = self.mask
+        c = self._frozen_cache(mask)
+        return c['slices'] if c is not None else _plane_slice(mask)
 
-    def phasor_tiles(self):
+    def phasor_tiles(self, mask=None):
         """Per-segment (slice, amplitude, opd) with the segment mask applied (plane.py:496-506).
 
-        amplitude/opd are returned as given (NumPy, possibly 0-d scalars).
+        amplitude/opd are returned as given (NumPy, possibly 0-d scalars);
+        ``mask`` is this plane's mask when the caller has already read it.
         """
-        mask_all = self.mask
+        mask_all = self.mask if mask is None else mask
+        c = self._frozen_cache(mask_all)
+        slices = c['slices'] if c is not None else _plane_slice(mask_all)
         amp_all, opd_all = self.amplitude, self.opd
         tiles = []
-        for n, s in enumerate(self._slice):
+        for n, s in enumerate(slices):
             mask = mask_all if self.size == 1 else mask_all[n]
             amp = amp_all if amp_all.size == 1 else amp_all[s] * mask[s]
             opd = opd_all if opd_all.size == 1 else opd_all[s]
@@ -196,11 +215,14 @@ class Plane(_PlaneBase):
         return tiles
 
     def _device_tiles(self):
-        """Device copies of per-slice (amp, opd) as float64; cached while frozen."""
-        if self.frozen and self._cache is not None:
-            return self._cache['tiles']
+        """Device copies of per-slice (amp, opd) as float64; cached while
+        frozen (see ``_frozen_cache``)."""
+        mask = self.mask
+        c = self._frozen_cache(mask)
+        if c is not None:
+            return c['tiles']
         out = []
-        for n, s, amp, opd in self.phasor_tiles():
+        for n, s, amp, opd in self.phasor_tiles(mask):
             amp = np.asarray(amp)
             opd = np.asarray(opd)
             if amp.size == 1 and opd.size == 1:
@@ -214,7 +236,7 @@ class Plane(_PlaneBase):
             to = to_dev(np.ascontiguousarray(o, dtype=np.float64))
             out.append((n, s, a if complex_amp else None, None, ta, to))
         if self.frozen:
-            self._cache = {'slices': _plane_slice(self.mask), 'tiles': out}
+            self._cache = {'mask': mask, 'slices': [t[1] for t in out], 'tiles': out}
         return out
 
     @property

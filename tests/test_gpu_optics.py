@@ -265,13 +265,29 @@ def _small_segmented(L, prop_shape=None, seg_tilts=True):
 
 
 @pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
-def test_segmented_plan_vs_oracle(L, precision):
-    # coherent fast path: all tiles share one output grid
+@pytest.mark.parametrize('contraction', ['per_tile', 'merged'])
+def test_segmented_plan_vs_oracle(L, precision, contraction):
+    # coherent fast path: all tiles share one output grid, contracted per
+    # tile (K concatenation) or summed into their union box
     p, wl, wt, ref = _small_segmented(L, seg_tilts=False)
     plan = L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, tilts=[L.Tilt(0.3e-6, -0.2e-6)],
-                     precision=precision, lambda_group=4)
+                     precision=precision, lambda_group=4, contraction=contraction)
     assert plan.info['coherent_fast'] == 1
+    assert plan.merged == (contraction == 'merged')
+    assert len(plan.tile_shapes) == (1 if plan.merged else 6)
     assert rel_l2(plan().double().cpu().numpy(), ref) < TOL_I
+
+
+def test_merge_refused_when_segments_split(L):
+    """Per-segment tilts give each segment its own output grid: a merged
+    contraction would not be exact, so 'merged' raises and 'auto' keeps the
+    tiles."""
+    p, wl, wt, _ = _small_segmented(L, seg_tilts=True)
+    with pytest.raises(ValueError):
+        L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, contraction='merged')
+    plan = L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2)
+    assert not plan.merged and len(plan.tile_shapes) == 6
+    plan.close()
 
 
 @pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
@@ -298,7 +314,7 @@ def test_plan_schedules(L, tile_n, lambda_group):
     coherent paths)."""
     p, wl, wt, ref = _small_segmented(L, seg_tilts=False)
     plan = L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, tilts=[L.Tilt(0.3e-6, -0.2e-6)],
-                     precision='bf16x3', lambda_group=lambda_group, tile_n=tile_n)
+                     precision='bf16x3', lambda_group=lambda_group, tile_n=tile_n, contraction='per_tile')
     assert plan.info['coherent_fast'] == 1
     assert rel_l2(plan().double().cpu().numpy(), ref) < TOL_I
     p, wl, wt, ref = _small_segmented(L, prop_shape=24, seg_tilts=True)
@@ -404,7 +420,7 @@ def test_plan_fields_coherent_groups(L):
     from oracle import mft
     p, wl, wt, _ = _small_segmented(L, seg_tilts=False)
     plan = L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, tilts=[L.Tilt(0.3e-6, -0.2e-6)],
-                     precision='fp32', lambda_group=4)
+                     precision='fp32', lambda_group=4, contraction='per_tile')
     assert plan.info['coherent_fast'] == 1
     dst = [torch.zeros((q['M'], q['N']), dtype=torch.complex64, device='cuda') for q in plan.problems]
     scale = [1.0 + 0.1 * k for k in range(plan.n_problems)]

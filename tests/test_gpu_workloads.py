@@ -40,38 +40,40 @@ def _oracle(cfg, wl, wt):
 def seg18():
     from lentil_amd import workloads
     cfg = workloads.segmented_18()
-    idx = np.arange(0, len(cfg['wavelengths']), 4)          # 8 of the 32 wavelengths
-    wl, wt = cfg['wavelengths'][idx], cfg['weights'][idx]
+    wl, wt = cfg['wavelengths'], cfg['weights']             # all 32 wavelengths
+    print('segmented_18: oracle image of 32 wavelengths', flush=True)
     return cfg, wl, wt, _oracle(cfg, wl, wt)
 
 
 @pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
 @pytest.mark.parametrize('tile_n', [64, 128])
-def test_segmented_18(L, seg18, precision, tile_n):
-    """18 uneven hex tiles (one Field each) x 8 wavelengths in one coherent
-    K-concatenated launch per stage."""
+@pytest.mark.parametrize('contraction', ['per_tile', 'merged'])
+def test_segmented_18(L, seg18, precision, tile_n, contraction):
+    """The full 32-wavelength image against the oracle, both ways of
+    contracting the 18 coherent segments: 18 uneven hex tiles (one Field
+    each) x 32 wavelengths = 576 MFTs in one K-concatenated launch per stage,
+    or the tiles summed into their union box, one MFT per wavelength."""
     cfg, wl, wt, ref = seg18
-    plan = _plan(L, cfg, wl, wt, precision, tile_n=tile_n, lambda_group=4)
+    plan = _plan(L, cfg, wl, wt, precision, tile_n=tile_n, lambda_group=8, contraction=contraction)
     assert plan.info['coherent_fast'] == 1
-    assert plan.n_problems == 18 * len(wl)
+    assert plan.merged == (contraction == 'merged')
+    assert plan.n_problems == (1 if plan.merged else 18) * len(wl)
     img = plan().double().cpu().numpy()
     err = rel_l2(img, ref)
-    print(f'segmented_18 {precision} tile_n={tile_n}: rel-L2 {err:.2e}', flush=True)
+    print(f'segmented_18 {precision} tile_n={tile_n} {contraction}: rel-L2 {err:.2e}', flush=True)
     assert err < TOL_I
 
 
-def test_segmented_18_full_sweep(L, seg18):
-    """All 32 wavelengths (576 tile x wavelength MFTs in one launch per stage):
-    the autotuned bf16x3 plan equals the fp32 plan within the gate."""
-    from lentil_amd import workloads
-    cfg = workloads.segmented_18()
-    wl, wt = cfg['wavelengths'], cfg['weights']
+def test_segmented_18_autotuned(L, seg18):
+    """The default plan (contraction chosen by cost, schedule by the
+    autotuner) merges the segments and meets the gate on all 32 wavelengths;
+    its cost model counts fewer MACs than the per-tile form."""
+    cfg, wl, wt, ref = seg18
     a = _plan(L, cfg, wl, wt, 'bf16x3', autotune=True)
-    assert a.n_problems == 576
-    b = _plan(L, cfg, wl, wt, 'fp32')
-    ia, ib = a().double().cpu().numpy(), b().double().cpu().numpy()
-    err = rel_l2(ia, ib)
-    print(f'segmented_18 full: bf16x3 vs fp32 rel-L2 {err:.2e}; tuned {a.lambda_group}, {a.tile_n}', flush=True)
+    assert a.merged and a.contraction_cost['merged'] < a.contraction_cost['per_tile']
+    ia = a().double().cpu().numpy()
+    err = rel_l2(ia, ref)
+    print(f'segmented_18 autotuned: rel-L2 {err:.2e}; tuned {a.lambda_group}, {a.tile_n}', flush=True)
     assert err < TOL_I
     assert np.all(ia >= 0)
 
@@ -80,18 +82,22 @@ def test_segmented_18_full_sweep(L, seg18):
 def stress():
     from lentil_amd import workloads
     cfg = workloads.stress_2048()
-    idx = np.array([0, len(cfg['wavelengths']) - 1])        # 450 and 950 nm
+    idx = np.array([0, 42, 85, len(cfg['wavelengths']) - 1])   # 450 nm to 950 nm
     wl, wt = cfg['wavelengths'][idx], cfg['weights'][idx]
+    print('stress_2048: oracle image of 4 wavelengths', flush=True)
     return cfg, wl, wt, _oracle(cfg, wl, wt)
 
 
 @pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
-def test_stress_2048_vs_oracle(L, stress, precision):
+@pytest.mark.parametrize('contraction', ['per_tile', 'merged'])
+def test_stress_2048_vs_oracle(L, stress, precision, contraction):
     """2048^2 pupil, 6 segment tiles, 4096^2 output with the off-axis
-    (37.3, -12.6) px shift, at the two extreme wavelengths: phases of hundreds
-    of turns, every tile merged coherently."""
+    (37.3, -12.6) px shift, at four wavelengths from 450 to 950 nm: phases of
+    hundreds of turns, every tile merged coherently (per tile, or as one
+    union-box MFT)."""
     cfg, wl, wt, ref = stress
-    plan = _plan(L, cfg, wl, wt, precision)
+    plan = _plan(L, cfg, wl, wt, precision, contraction=contraction)
+    assert plan.merged == (contraction == 'merged')
     img = plan().double().cpu().numpy()
     err = rel_l2(img, ref)
     print(f'stress_2048 {precision}: rel-L2 {err:.2e}', flush=True)
@@ -104,11 +110,12 @@ def test_stress_2048_vs_oracle(L, stress, precision):
 
 def test_stress_2048_full_sweep(L):
     """All 128 wavelengths from a hipGraph: replays equal the eager result and
-    the bf16x3 image equals the fp32 image within the gate."""
+    the merged bf16x3 image equals the per-tile fp32 image within the gate."""
     from lentil_amd import workloads
     cfg = workloads.stress_2048()
     wl, wt = cfg['wavelengths'], cfg['weights']
     a = _plan(L, cfg, wl, wt, 'bf16x3', use_graph=True)
+    assert a.merged
     out = torch.empty(a.shape_out, dtype=torch.float32, device='cuda')
     first = a(out).clone()          # captures the graph for this image
     for _ in range(2):
@@ -116,9 +123,9 @@ def test_stress_2048_full_sweep(L):
         a(out)                      # replays
         torch.cuda.synchronize()
         assert torch.equal(out, first)
-    b = _plan(L, cfg, wl, wt, 'fp32', use_graph=False)
+    b = _plan(L, cfg, wl, wt, 'fp32', use_graph=False, contraction='per_tile')
     err = rel_l2(first.double().cpu().numpy(), b().double().cpu().numpy())
-    print(f'stress_2048 full sweep: bf16x3 vs fp32 rel-L2 {err:.2e}', flush=True)
+    print(f'stress_2048 full sweep: merged bf16x3 vs per-tile fp32 rel-L2 {err:.2e}', flush=True)
     assert err < TOL_I
     a.close()
     b.close()

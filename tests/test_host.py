@@ -186,3 +186,162 @@ def test_bench_refuses_several_gpus():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert 'one GPU' in msg['error']
+
+
+def test_null_handles_are_rejected():
+    """Every entry point that takes a context, plan or MFT-plan handle first
+    returns MFTX_EHANDLE for a null handle, before touching any other
+    argument or the device (SURVEY: invalid handle is host-forcible)."""
+    if not os.path.exists(LIB):
+        pytest.skip('libmftx.so not built (run __graft_entry__.build())')
+    from lentil_amd import _lib
+    lib = _lib.load()
+    skip = {'mftx_version', 'mftx_create', 'mftx_last_error'}
+    checked = 0
+    for name, (_, argtypes) in _lib.SIGNATURES.items():
+        if name in skip:
+            continue
+        args = []
+        for t in argtypes:
+            if t in (ctypes.c_double,):
+                args.append(0.0)
+            elif t in (ctypes.c_int, ctypes.c_int32, ctypes.c_int64):
+                args.append(0)
+            else:
+                args.append(None)
+        assert getattr(lib, name)(*args) == _lib.EHANDLE, name
+        checked += 1
+    assert checked == len(_lib.SIGNATURES) - len(skip)
+    assert lib.mftx_last_error(None) == b'null context'
+    # a null output pointer is an argument error, checked before the device
+    assert lib.mftx_create(0, None) == _lib.EINVAL
+
+
+class _FreezePlane:
+    """tests/test_plane.py:183-194: a plane whose OPD is random on every read."""
+
+    @staticmethod
+    def make():
+        import lentil_amd as L
+
+        class FreezePlane(L.Plane):
+            def __opd__(self):
+                return np.random.uniform()
+        return FreezePlane()
+
+
+def test_freeze():
+    p = _FreezePlane.make()
+    p.freeze()
+    a = p.opd
+    b = p.opd
+    assert a == b
+    with pytest.raises(RuntimeError):
+        p.freeze()
+    with pytest.raises(RuntimeError):
+        p.opd = 1.0
+    p.thaw()
+    assert not p.frozen
+    assert p.opd != p.opd  # read through the hook again
+    # freeze(inplace=False) returns a frozen copy and leaves the plane alone
+    q = p.freeze(inplace=False)
+    assert q.frozen and not p.frozen
+    assert q.opd == q.opd
+
+
+def test_plane_overload_methods():
+    """tests/test_plane.py:40-59: the __amp__/__opd__/__mask__ hooks are what
+    the properties return."""
+    import lentil_amd as L
+
+    class Plane(L.Plane):
+        def __amp__(self):
+            return np.array(1)
+
+        def __opd__(self):
+            return np.array(2)
+
+        def __mask__(self):
+            return np.array(3)
+
+    p = Plane()
+    assert p.amplitude == 1 and p.opd == 2 and p.mask == 3
+
+
+def test_frozen_slices_follow_the_mask():
+    """A frozen plane's cached slices are used only for the mask they were
+    made from; another mask (a hook returning a new array) gets its own."""
+    import lentil_amd as L
+    m1 = np.zeros((2, 16, 16))
+    m1[0, 2:6, 3:9] = 1
+    m1[1, 9:14, 1:4] = 1
+    m2 = np.zeros((2, 16, 16))
+    m2[0, 1:3, 1:3] = 1
+    m2[1, 10:16, 8:16] = 1
+
+    class MaskPlane(L.Plane):
+        current = m1
+
+        def __mask__(self):
+            return self.current
+
+    p = MaskPlane(amplitude=np.ones((16, 16)), opd=np.zeros((16, 16)), pixelscale=1)
+    p.freeze()
+    p._cache = {'mask': m1, 'slices': ['cached'], 'tiles': []}
+    assert p._slice == ['cached']
+    p.current = m1.copy()       # an equal array: still the cache
+    assert p._slice == ['cached']
+    p.current = m2              # another mask: its own slices
+    assert p._slice == [optics.boundary_slice(m) for m in m2]
+    p.thaw()
+    assert p._cache is None
+    assert p._slice == [optics.boundary_slice(m) for m in m2]
+
+
+def test_merged_contraction_is_exact():
+    """The union-box tile of a segmented pupil, transformed once, equals the
+    coherent sum of its segment tiles transformed one by one (complex128
+    oracle MFTs on the plan geometry, every wavelength), and lands on the
+    same output grid; per-segment tilts make the merge inexact, so it is
+    refused."""
+    import lentil_amd as L
+    from lentil_amd.batched import plan_geometry
+    from oracle import mft
+    rng = np.random.default_rng(3)
+    masks = inputs.hex_segments(rings=1, seg_radius=9, seg_gap=1, drop=(0,))
+    n = masks.shape[-1]
+    amp = np.sum(masks, axis=0) * rng.uniform(0.8, 1.0, masks.shape[-2:])
+    opd = inputs.zernike_compose(np.sum(masks, axis=0), np.r_[0, rng.normal(0, 60e-9, size=8)])
+    p = L.Pupil(amplitude=amp, opd=opd, mask=masks, pixelscale=1 / n, focal_length=10)
+    wl = np.array([600e-9, 710e-9, 830e-9])
+    args = (p, wl, None, 5e-6, 24, None, 2, None, [L.Tilt(0.4e-6, -0.25e-6)])
+    gt = plan_geometry(*args, contraction='per_tile')
+    gm = plan_geometry(*args, contraction='merged')
+    ga = plan_geometry(*args)
+    assert not gt['merged'] and len(gt['tiles']) == len(masks)
+    assert gm['merged'] and len(gm['tiles']) == 1 and len(gm['problems']) == len(wl)
+    assert ga['merged'] == (ga['cost_merged'] < ga['cost_per_tile'])
+    (ua, uo), = gm['tiles']
+    for w, lam in enumerate(wl):
+        pm = gm['problems'][w]
+        ref = np.zeros((pm['M'], pm['N']), dtype=complex)
+        for q in (q for q in gt['problems'] if q['wave'] == w):
+            a, o = gt['tiles'][q['tile']]
+            assert (q['M'], q['N'], q['alpha'], q['shift'], q['out_r0'], q['out_c0']) == \
+                (pm['M'], pm['N'], pm['alpha'], pm['shift'], pm['out_r0'], pm['out_c0'])
+            ref += mft.dft2(a * np.exp(2j * np.pi * o / lam), q['alpha'], (q['M'], q['N']), q['shift'], q['offset'])
+        got = mft.dft2(ua * np.exp(2j * np.pi * uo / lam), pm['alpha'], (pm['M'], pm['N']), pm['shift'],
+                       pm['offset'])
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+    # a planted mistake (the union box placed one row off) is caught
+    bad = mft.dft2(ua * np.exp(2j * np.pi * uo / wl[0]), pm['alpha'], (pm['M'], pm['N']), pm['shift'],
+                   (gm['problems'][0]['offset'][0] + 1, gm['problems'][0]['offset'][1]))
+    ref0 = sum(mft.dft2(gt['tiles'][q['tile']][0] * np.exp(2j * np.pi * gt['tiles'][q['tile']][1] / wl[0]),
+                        q['alpha'], (q['M'], q['N']), q['shift'], q['offset'])
+               for q in gt['problems'] if q['wave'] == 0)
+    assert np.linalg.norm(bad - ref0) / np.linalg.norm(ref0) > 1e-3
+    # per-segment tilts split the segments' output grids: no merge
+    p.tilt = [L.Tilt(1e-6 * k, -0.5e-6 * k) for k in range(len(masks))]
+    assert not plan_geometry(*args)['merged']
+    with pytest.raises(ValueError):
+        plan_geometry(*args, contraction='merged')
