@@ -318,10 +318,15 @@ int mftx_plan_execute_fields(mftx_plan* plan);
  * on later executes with the same pointer (use_graph != 0), or run eagerly. */
 int mftx_plan_set_graph(mftx_plan* plan, int32_t use_graph);
 int mftx_plan_get_info(const mftx_plan* plan, mftx_plan_info* info);
-/* Run the plan's launch groups eagerly `repeats` times with HIP events
- * recorded on the context stream between them; ms[0..3] receive the mean
- * milliseconds of (phasors, stage-1 GEMM, stage-2 GEMM, detector reduction).
- * The image is overwritten.  Measurement hook; no reference counterpart. */
+/* Run the plan's launch groups `repeats` times back to back (at most 4096)
+ * with HIP events between them, in the mode executes run in: when the plan
+ * replays a graph, all repeats are captured in one graph whose event-record
+ * nodes time them (replayed once to warm up, then timed); otherwise they are
+ * queued eagerly on the context stream.  ms[0..3] receive the mean
+ * milliseconds of (phasors, stage-1 GEMM, stage-2 GEMM, detector reduction),
+ * ms[4] is 1 when the graph mode was used and 0 when the launches ran eagerly
+ * (also when the runtime refused event nodes in a graph).  The image is
+ * overwritten.  Measurement hook; no reference counterpart. */
 int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi,
                       int32_t repeats, double* ms);
 int mftx_plan_destroy(mftx_plan* plan);

@@ -364,15 +364,18 @@ class PSFPlan:
 
     def profile(self, out=None, repeats=5):
         """Mean milliseconds of (phasors, stage-1 GEMM, stage-2 GEMM, reduction),
-        timed with HIP events on the launch stream."""
+        timed with HIP events on the launch stream over ``repeats`` back-to-back
+        executes in the mode calls run in (graph replay when ``use_graph``);
+        ``mode`` says which mode the events timed."""
         if out is None:
             out = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
-        ms = (ctypes.c_double * 4)()
+        ms = (ctypes.c_double * 5)()
         ctx = _lib.context()
         rc = _lib.load().mftx_plan_profile(self._plan, ctypes.c_void_p(out.data_ptr()), _lib.dtype_code(out),
                                            out.stride(0), int(repeats), ms)
         _lib.check(rc, ctx, 'mftx_plan_profile')
-        return dict(phasor=ms[0], stage1=ms[1], stage2=ms[2], reduce=ms[3])
+        return dict(phasor=ms[0], stage1=ms[1], stage2=ms[2], reduce=ms[3],
+                    mode='graph replay' if ms[4] else 'eager')
 
     def set_opd(self, tile, opd, copy=0):
         """Overwrite a tile's OPD (of one copy) in place (the plan re-reads it on every call)."""

@@ -89,6 +89,14 @@ class _PlaneBase:
 
     @property
     def mask(self):
+        # A frozen plane with no mask of its own and no __mask__ override has
+        # the mask of its frozen amplitude (plane.py:42-46), so it is derived
+        # once per freeze instead of on every read.
+        if self._frozen and self._mask is None and type(self).__mask__ is _PlaneBase.__mask__:
+            m = self.__dict__.get('_derived_mask')
+            if m is None:
+                m = self._derived_mask = self.__mask__()
+            return m
         return self.__mask__()
 
     @property
@@ -128,18 +136,20 @@ class _PlaneBase:
             setattr(obj, f'_{attr}', getattr(obj, attr))
         obj._frozen = True
         obj._cache = None
+        obj._derived_mask = None
         if not inplace:
             return obj
 
     def thaw(self):
         self._frozen = False
         self._cache = None
+        self._derived_mask = None
 
     def __deepcopy__(self, memo):
         cls = self.__class__
         new = cls.__new__(cls)
         for k, v in self.__dict__.items():
-            setattr(new, k, None if k == '_cache' else copy.deepcopy(v, memo))
+            setattr(new, k, None if k in ('_cache', '_derived_mask') else copy.deepcopy(v, memo))
         return new
 
 
@@ -207,17 +217,19 @@ class Plane(_PlaneBase):
         slices = c['slices'] if c is not None else _plane_slice(mask_all)
         amp_all, opd_all = self.amplitude, self.opd
         tiles = []
+        single = mask_all.ndim in (0, 1, 2)
         for n, s in enumerate(slices):
-            mask = mask_all if self.size == 1 else mask_all[n]
+            mask = mask_all if single else mask_all[n]
             amp = amp_all if amp_all.size == 1 else amp_all[s] * mask[s]
             opd = opd_all if opd_all.size == 1 else opd_all[s]
             tiles.append((n, s, amp, opd))
         return tiles
 
-    def _device_tiles(self):
+    def _device_tiles(self, mask=None):
         """Device copies of per-slice (amp, opd) as float64; cached while
-        frozen (see ``_frozen_cache``)."""
-        mask = self.mask
+        frozen (see ``_frozen_cache``).  ``mask`` is this plane's mask when
+        the caller has already read it."""
+        mask = self.mask if mask is None else mask
         c = self._frozen_cache(mask)
         if c is not None:
             return c['tiles']
@@ -308,12 +320,14 @@ class Plane(_PlaneBase):
                             f"'{self.ptype}'")
         from .wavefront import Wavefront
         pixelscale = _mul_pixelscale(self.pixelscale, wavefront.pixelscale)
-        shape = wavefront.shape if self.shape == () else self.shape
+        mask = self.mask  # read once: shape, size and the slices all follow from it
+        plane_shape = mask.shape if mask.ndim in (0, 1, 2) else (mask.shape[1], mask.shape[2])
+        shape = wavefront.shape if plane_shape == () else plane_shape
         ptype = _mul_ptype_table[wavefront.ptype][self.ptype]
         out = Wavefront.empty(wavelength=wavefront.wavelength, pixelscale=pixelscale,
                               focal_length=wavefront.focal_length, shape=shape, ptype=ptype)
         phasors = []
-        for n, s, camp, sopd, ta, to in self._device_tiles():
+        for n, s, camp, sopd, ta, to in self._device_tiles(mask):
             if ta is None:  # scalar amplitude and OPD
                 data = camp * np.exp(2 * np.pi * 1j * sopd / wavefront.wavelength)
             else:
@@ -322,7 +336,7 @@ class Plane(_PlaneBase):
                     ph = torch.from_numpy(np.ascontiguousarray(np.exp(1j * np.angle(camp))).astype(np.complex64))
                     data = data * ph.to(data.device)
             phasors.append(Field(data=data, pixelscale=self.pixelscale,
-                                 offset=helper.slice_offset(s, self.shape),
+                                 offset=helper.slice_offset(s, plane_shape),
                                  tilt=[self.tilt[n]] if self.tilt else []))
         for fld in wavefront.data:
             for ph in phasors:

@@ -215,6 +215,27 @@ def test_plan_graph_replays(L):
     assert torch.allclose(acc, 2 * eager, rtol=1e-6, atol=0)
 
 
+def test_plan_profile_modes(L):
+    """profile() times the launch groups back to back in the mode calls run
+    in (a graph of repeats for a graph plan when the runtime can time event
+    nodes in a graph, eager launches otherwise), and the image it leaves is
+    the plan's image."""
+    from lentil_amd import workloads
+    cfg = workloads.mono_256()
+    p = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'], focal_length=10)
+    wl = np.linspace(600e-9, 700e-9, 6)
+    for use_graph, mode in ((True, 'graph replay'), (False, 'eager')):
+        plan = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, use_graph=use_graph, lambda_group=4)
+        ref = plan().clone()
+        out = torch.full_like(ref, -1.0)
+        ms = plan.profile(out, repeats=3)
+        torch.cuda.synchronize()
+        assert ms['mode'] in (mode, 'eager')
+        assert all(ms[k] > 0 for k in ('phasor', 'stage1', 'stage2', 'reduce'))
+        assert torch.equal(out, ref)
+        plan.close()
+
+
 def _c2_reference():
     from lentil_amd import workloads
     cfg = workloads.broadband_512()
