@@ -79,6 +79,26 @@ int mftx_dft2(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, i
               double offset_r, double offset_c, double scale, int conj, int precision);
 
 /*
+ * mftx_dft2 without any device allocation.  No counterpart in the
+ * reference: this is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as mftx_dft2, but the
+ * packed operands live in a fixed array of MFTX_NOALLOC_SCRATCH_BYTES of
+ * device memory that the library holds on every GPU it runs on until the
+ * process ends (its address is looked up once per device, at mftx_create).
+ * A transform whose operands need more than that (about a 1024^2 -> 2048^2
+ * transform) fails with MFTX_EINVAL and a message giving both sizes.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (other host threads, or streams with no order between them) race on it;
+ * calls queued on one stream are ordered and safe.
+ */
+#define MFTX_NOALLOC_SCRATCH_BYTES (32u << 20)
+int mftx_dft2_noalloc(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int64_t n, int64_t ldf,
+                      void* F, int64_t M, int64_t N, int64_t ldF,
+                      double alpha_r, double alpha_c, double shift_r, double shift_c,
+                      double offset_r, double offset_c, double scale, int conj, int precision);
+
+/*
  * Pupil phasor  out = amp * exp(2*pi*i*opd/wavelength)  (complex64, m x n, dense).
  * Replaces the phasor of Plane.__mul__ (plane.py:503-510).  amp and opd are
  * float32 or float64 (dtype codes 1 or 3) with row strides ld_amp, ld_opd.

@@ -67,6 +67,8 @@ SIGNATURES = {
     'mftx_synchronize': (c_int, [c_vp]),
     'mftx_dft2': (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_i64,
                           c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_int, c_int]),
+    'mftx_dft2_noalloc': (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_i64,
+                                  c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_int, c_int]),
     'mftx_phasor': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, c_i64, c_dbl, c_vp]),
     'mftx_insert': (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_int, c_i64,
                             c_i64, c_i64, c_int, c_dbl]),

@@ -707,6 +707,12 @@ int mftx_create(int device, mftx_ctx** out) {
       return MFTX_EDEVICE;
     }
   }
+  // the no-allocation entry point's scratch address on this device
+  void* scratch = nullptr;
+  if (scratch_address(device, &scratch) != hipSuccess) {
+    mftx_destroy(ctx);
+    return MFTX_EDEVICE;
+  }
 
   *out = ctx;
   return MFTX_OK;

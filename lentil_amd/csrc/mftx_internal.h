@@ -221,5 +221,8 @@ int dft2_on_workspace(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m, int6
 int set_error(mftx_ctx* ctx, int code, const char* msg);
 // the device a context was created on
 int context_device(const mftx_ctx* ctx);
+// device address of the no-allocation entry point's scratch array on `device`
+// (current), looked up once per device (mftx_noalloc.hip)
+hipError_t scratch_address(int device, void** out);
 
 }  // namespace mftx
