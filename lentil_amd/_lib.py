@@ -15,7 +15,9 @@ import threading
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'libmftx.so')
+# MFTX_LIBRARY names another build of the library (e.g. to compare two builds
+# on one device); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get('MFTX_LIBRARY') or os.path.join(HERE, 'libmftx.so')
 
 OK, EINVAL, ENOMEM, EDEVICE, EHANDLE = 0, -1, -2, -3, -4
 DT_C64, DT_F32, DT_C128, DT_F64 = 0, 1, 2, 3

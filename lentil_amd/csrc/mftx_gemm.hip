@@ -57,14 +57,19 @@ template <int PREC, int WN, int KS> struct Lds;
 
 // bf16x3: 4 planes of [row][16 KS] bf16 (A: re_hi, re_lo, im_hi, im_lo; B:
 // cos_hi, cos_lo, sin_hi, sin_lo), made of 16-byte units of 8 K values.  Units
-// are swizzled per row, unit' = unit ^ swz_key(row), so the 16-byte accesses of
-// 16 consecutive lanes (16 consecutive rows, one unit) cover all 64 banks once.
+// are swizzled per row, unit' = unit ^ swz_key(row).  For KS = 2 (64-byte
+// rows) the key ((row >> 1) ^ (row >> 2)) & 3 keeps the 16-byte accesses of
+// 8 consecutive rows on distinct banks of a 32-bank (128-byte) LDS line and
+// those of 16 consecutive rows on distinct banks of a 64-bank line, so the
+// fragment reads and the A staging stores are conflict-free whichever way the
+// hardware groups the lanes of a 16-byte access (checked on the CPU for 8 and
+// 16 lanes per cycle); (row >> 2) & 3 was conflict-free only for 16.
 template <int WN, int KS> struct Lds<PREC_BF16X3, WN, KS> {
   __bf16 a[2][4][BM][16 * KS];
   __bf16 b[2][4][Cfg<WN, KS>::BS][16 * KS];
 };
 template <int KS> __device__ __forceinline__ int swz_key(int row) {
-  return KS == 1 ? (row >> 3) & 1 : (row >> 2) & 3;
+  return KS == 1 ? (row >> 3) & 1 : ((row >> 1) ^ (row >> 2)) & 3;
 }
 // element offset of logical unit u of a row
 template <int KS> __device__ __forceinline__ int swz(int row, int u) { return 8 * (u ^ swz_key<KS>(row)); }
@@ -87,13 +92,25 @@ typedef __attribute__((address_space(1))) const u32x4 gu32x4;  // global-memory 
 // is row (i & 15) + 16 ((i >> 6) & 7), part (i >> 4) & 3, K step i >> 9, so
 // 16 consecutive lanes store 16 consecutive rows of one part (conflict-free
 // LDS stores).  Thread t stages items t + j * NT.
-// B generation: BS x 8 items (slot, group of TW K values), one per thread:
-// slot (t & 15) + 16 (t >> 7), group (t >> 4) & 7.
+// B generation: BS x 8 items (slot, group of TW K values), one per thread.
+// KS = 1: slot (t & 15) + 16 (t >> 7), group (t >> 4) & 7.  KS = 2 (8-byte
+// stores of 4 K values): lane l of wave w takes slot (l & 7) + 8 ((l >> 4) & 1)
+// + 16 (w >> 1) and group ((l >> 3) & 1) + 2 ((l >> 5) & 1) + 4 (w & 1), so 16
+// consecutive lanes store 8 slots x the two halves of one 16-byte unit: 128
+// bytes on distinct banks (and 32 lanes 256 bytes), with the KS = 2 key.
 __device__ __forceinline__ int item_row(int i) { return (i & 15) + 16 * ((i >> 6) & 7); }
 __device__ __forceinline__ int item_part(int i) { return (i >> 4) & 3; }
 __device__ __forceinline__ int item_kstep(int i) { return i >> 9; }
-__device__ __forceinline__ int bitem_slot(int t) { return (t & 15) + 16 * (t >> 7); }
-__device__ __forceinline__ int bitem_group(int t) { return (t >> 4) & 7; }
+template <int KS> __device__ __forceinline__ int bitem_slot(int t) {
+  if (KS == 1) return (t & 15) + 16 * (t >> 7);
+  const int l = t & 63;
+  return (l & 7) + 8 * ((l >> 4) & 1) + 16 * (t >> 7);
+}
+template <int KS> __device__ __forceinline__ int bitem_group(int t) {
+  if (KS == 1) return (t >> 4) & 7;
+  const int l = t & 63;
+  return ((l >> 3) & 1) + 2 * ((l >> 5) & 1) + 4 * ((t >> 6) & 1);
+}
 
 // Staged chunk in registers: A prefetch and the generated twiddles.
 // bf16x3: b[p] holds plane p of the TW K values (TW / 2 words); fp32: the TW
@@ -502,8 +519,8 @@ __device__ __forceinline__ void gemm_body(const Work& wk, const Problem* __restr
 
   // staging coordinates: A rows from a_row0, one slot / K group of twiddles
   const long long a_row0 = (long long)wk.tr * BM;
-  const int st_slot = bitem_slot(tid);
-  const int st_g = bitem_group(tid);
+  const int st_slot = bitem_slot<KS>(tid);
+  const int st_g = bitem_group<KS>(tid);
   const int b_slot = wk.tc * BS + st_slot;
 
   f32x16 acc[4], ip, im;
@@ -621,6 +638,241 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
     gemm_body<PREC, EPI, WN, KS, false>(wk, probs, chunk_x, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Warp-specialised form of the 128-row x 64-slot, 32-wide-K configuration
+// (BF16x3).  Twelve waves per workgroup, three per SIMD:
+//   consumers (waves 0-7): a 4 x 2 grid of 32-row x 32-slot blocks; each
+//     issues the direct-to-LDS loads (global_load_lds_dwordx4, 16 bytes per
+//     lane at M0 + 16 lane) of 16 rows of the A tile two chunks ahead, reads
+//     its fragments from LDS and issues the 24 MFMAs of a chunk, and runs the
+//     problem epilogues;
+//   producers (waves 8-11): producer wave p generates the twiddles of K
+//     group p (8 consecutive K values for each of the 64 slots) one chunk
+//     ahead into LDS.
+// The twiddle VALU work thus runs on a wave of its own on each SIMD while
+// that SIMD's two consumer waves keep the MFMA pipe busy.  The A operand
+// never passes through VGPRs, and its loads, which stall the issuing wave
+// while the vector memory path is backed up, sit with the MFMA waves (the
+// SIMD's other consumer keeps the pipe busy) rather than in series with the
+// twiddle VALU.  LDS: A in three buffers (chunk s in s % 3), B in two
+// (s % 2), 128 KiB.  An A buffer is row-major, a row's 256 bytes of the
+// chunk (two 16-wide K blocks of 4 planes) in 16-byte units swizzled as
+// unit ^ (row & 15): one load moves 4 whole rows (8 cache lines, every byte
+// used), and the fragment reads of 16 (or 8) consecutive rows fall on
+// distinct banks.  One barrier per chunk.  Consumers wait for their previous
+// chunk's loads (vmcnt(4): the 4 loads just issued may stay in flight; loads
+// complete in issue order, so epilogue stores in between do not weaken this)
+// and their fragment reads, producers for their twiddle stores.  The barrier
+// and waits are written out: an LDS fence would make the compiler drain every
+// load, direct-to-LDS ones included, at each barrier.
+namespace ws {
+constexpr int NCONS = 8, NPROD = 4;
+constexpr int NT = 64 * (NCONS + NPROD);
+constexpr int BS = 64, BKC = 32;
+constexpr int NA = 3, NB = 2;
+struct Lds {
+  __bf16 a[NA][BM][128];   // [row][16-byte unit ^ (row & 15)][8 K values]
+  __bf16 b[NB][4][BS][32];
+};
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) const void gvoid;
+}  // namespace ws
+
+// A consumer's A cursor: the chunk its next direct-to-LDS loads fetch.
+// Consumer wave w loads rows 16 w .. 16 w + 15 of the tile, 4 rows per load:
+// lane l takes row 16 w + 4 j + (l >> 4) of load j and LDS unit l & 15, which
+// holds the row's logical unit (l & 15) ^ (row & 15) (row & 15 = 4 j + (l >> 4)).
+struct WsACursor {
+  int p, c, nch;              // problem, chunk in it, chunks in it
+  const char* base;           // this lane's row of load 0 at chunk 0
+  long long row4;             // bytes between the rows of consecutive loads
+  int unit[4];                // source byte offset in the row for load j
+};
+
+__device__ __forceinline__ void ws_a_set(WsACursor& ac, const Problem& P, long long row0, int lane, int w) {
+  const long long ldb = P.lda * 2;  // bytes per A row
+  const int rr = lane >> 4, pu = lane & 15;
+  ac.nch = P.kpad / ws::BKC;
+  ac.row4 = 4 * ldb;
+  ac.base = (const char*)P.a + (row0 + 16 * w + rr) * ldb;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ac.unit[j] = 16 * (pu ^ (4 * j + rr));
+}
+
+// Direct-to-LDS loads of this wave's 16 rows of the cursor's chunk into A buffer ab.
+__device__ __forceinline__ void ws_a_load(ws::Lds& L, int ab, int w, const WsACursor& ac) {
+  const char* src = ac.base + (long long)ac.c * 256;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    __builtin_amdgcn_global_load_lds((ws::gvoid*)(src + j * ac.row4 + ac.unit[j]),
+                                     (ws::lds_void*)&L.a[ab][16 * w + 4 * j][0], 16, 0, 0);
+}
+
+// The producer's twiddle state for one problem: 8 consecutive K values of one
+// slot per chunk, t_i = cis(alpha x Y) cis(alpha i Y), i = 0..7.
+struct WsB {
+  int p, c, nch, cb;
+  double alpha4, y;
+  float2 q[8];
+};
+
+__device__ __forceinline__ void ws_b_set(WsB& b, const Problem& P, int slot) {
+  b.alpha4 = 4.0 * P.alpha;
+  b.y = P.y0 + (double)slot;
+  b.nch = P.kpad / ws::BKC;
+  b.cb = P.chunk_begin;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) b.q[i] = cis_quarter(b.alpha4 * ((double)i * b.y));
+}
+
+// Twiddles of K group g (K 8g .. 8g+7 of the chunk) of this thread's slot into B buffer bb.
+__device__ __forceinline__ void ws_b_write(ws::Lds& L, int bb, int slot, int g, const WsB& b,
+                                           const double* __restrict__ chunk_x) {
+  // chunk_x holds one origin per 16-wide K block; this group lies in block g >> 1
+  const double* px = chunk_x + __builtin_amdgcn_readfirstlane(b.cb + 2 * b.c + (g >> 1));
+  const double x = uniform_double(px) + (double)(8 * (g & 1));
+  const float2 pf = cis_quarter(b.alpha4 * (x * b.y));
+  uint32_t ch[4], cl[4], sh[4], sl[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float2 t0 = cmul(pf, b.q[2 * i]), t1 = cmul(pf, b.q[2 * i + 1]);
+    split2(t0.x, t1.x, ch[i], cl[i]);
+    split2(t0.y, t1.y, sh[i], sl[i]);
+  }
+  const int e = swz<2>(slot, g);
+  *(u32x4*)&L.b[bb][0][slot][e] = u32x4{ch[0], ch[1], ch[2], ch[3]};
+  *(u32x4*)&L.b[bb][1][slot][e] = u32x4{cl[0], cl[1], cl[2], cl[3]};
+  *(u32x4*)&L.b[bb][2][slot][e] = u32x4{sh[0], sh[1], sh[2], sh[3]};
+  *(u32x4*)&L.b[bb][3][slot][e] = u32x4{sl[0], sl[1], sl[2], sl[3]};
+}
+
+template <int EPI>
+__device__ __forceinline__ void gemm_ws_body(const Work& wk, const Problem* __restrict__ probs,
+                                             const double* __restrict__ chunk_x, ws::Lds& L) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  int nchunks = 0;  // every problem's K loop back to back (uniform)
+  for (int p = wk.pbeg; p < wk.pend; ++p) nchunks += probs[p].kpad / ws::BKC;
+
+  if (wid >= ws::NCONS) {
+    // ---------------- producer ----------------
+    const int pw = wid - ws::NCONS;           // K group of the twiddles
+    const int slot = lane;                    // this thread's twiddle slot in the tile
+    WsB b;
+    b.p = wk.pbeg;
+    b.c = 0;
+    ws_b_set(b, probs[b.p], wk.tc * ws::BS + slot);
+    auto b_next = [&]() {
+      if (++b.c == b.nch) {
+        b.c = 0;
+        if (++b.p < wk.pend) ws_b_set(b, probs[b.p], wk.tc * ws::BS + slot);
+      }
+    };
+    ws_b_write(L, 0, slot, pw, b, chunk_x);
+    b_next();
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    for (int s = 0; s < nchunks; ++s) {
+      if (s + 1 < nchunks) {
+        ws_b_write(L, (s + 1) & 1, slot, pw, b, chunk_x);
+        b_next();
+      }
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    }
+    return;
+  }
+
+  // ---------------- consumer ----------------
+  const int wr = wid >> 1, wc = wid & 1;
+  int rows = 0;
+  for (int p = wk.pbeg; p < wk.pend; ++p) rows = max(rows, probs[p].rows);
+  const bool mf = wk.tr * BM + wr * 32 < rows;  // wave-uniform: past the last row a wave issues no MFMAs
+  f32x16 acc[4], ip, im;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
+  ip = f32x16{};
+  im = f32x16{};
+  const int r0 = wr * 32 + (lane & 31), c0 = wc * 32 + (lane & 31);
+  const long long row0 = (long long)wk.tr * BM;
+  WsACursor ac;
+  ac.p = wk.pbeg;
+  ac.c = 0;
+  ws_a_set(ac, probs[ac.p], row0, lane, wid);
+  // past the work's last chunk the cursor stays on it: the loads stay valid and nobody reads them
+  auto a_next = [&]() {
+    if (ac.c + 1 < ac.nch) {
+      ++ac.c;
+    } else if (ac.p + 1 < wk.pend) {
+      ++ac.p;
+      ac.c = 0;
+      ws_a_set(ac, probs[ac.p], row0, lane, wid);
+    }
+  };
+  ws_a_load(L, 0, wid, ac);
+  a_next();
+  ws_a_load(L, 1, wid, ac);
+  a_next();
+  asm volatile("s_waitcnt vmcnt(4)\n\ts_barrier" ::: "memory");
+  int pc = wk.pbeg, cs = 0, cn = probs[pc].kpad / ws::BKC;
+  int ab = 0, an = 2;  // A buffers of chunks s and s + 2
+  for (int s = 0; s < nchunks; ++s) {
+    ws_a_load(L, an, wid, ac);
+    a_next();
+    an = an == 2 ? 0 : an + 1;
+    if (mf) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int koff = swz<2>(lane & 31, 2 * j + (lane >> 5));
+        bf16x8_t a[4], bq[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          // logical unit of (K block j, plane p, half lane >> 5) in the row-major A buffer
+          const int u = (8 * j + 2 * p + (lane >> 5)) ^ (r0 & 15);
+          a[p] = *(const bf16x8_t*)&L.a[ab][r0][8 * u];
+          bq[p] = *(const bf16x8_t*)&L.b[s & 1][p][c0][koff];
+        }
+        sblock_bf16(a, bq, acc);
+      }
+    }
+    asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    ab = ab == 2 ? 0 : ab + 1;
+    if (++cs == cn) {
+      problem_epilogue<PREC_BF16X3, EPI>(probs[pc], wk, ws::BS, wr, wc, lane, acc, ip, im);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
+      if (++pc < wk.pend) {
+        cs = 0;
+        cn = probs[pc].kpad / ws::BKC;
+      }
+    }
+  }
+  // no load may still be writing LDS when the workgroup ends
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  if (EPI == EPI_INTENSITY) store_partial(wk, probs[wk.pbeg], ws::BS, wr, wc, lane, ip, im);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(ws::NT, 1) void gemm_ws_kernel(const Work* __restrict__ works,
+                                                            const Problem* __restrict__ probs,
+                                                            const double* __restrict__ chunk_x) {
+  __shared__ ws::Lds lds;
+  const Work wk = works[blockIdx.x];
+  if (wk.pbeg >= wk.pend) return;
+  gemm_ws_body<EPI>(wk, probs, chunk_x, lds);
+}
+
+static hipError_t launch_ws(int epi, const Work* work, int nwork, const Problem* probs, const double* chunk_x,
+                            hipStream_t s) {
+  dim3 grid(nwork), block(ws::NT);
+  if (epi == EPI_STORE_CPLX)
+    hipLaunchKernelGGL((gemm_ws_kernel<EPI_STORE_CPLX>), grid, block, 0, s, work, probs, chunk_x);
+  else if (epi == EPI_STORE_PACKED_T)
+    hipLaunchKernelGGL((gemm_ws_kernel<EPI_STORE_PACKED_T>), grid, block, 0, s, work, probs, chunk_x);
+  else
+    hipLaunchKernelGGL((gemm_ws_kernel<EPI_INTENSITY>), grid, block, 0, s, work, probs, chunk_x);
+  return hipGetLastError();
+}
+
 template <int WN, int KS>
 static hipError_t launch_cfg(int prec, int epi, const Work* work, int nwork, const Problem* probs,
                              const double* chunk_x, hipStream_t s) {
@@ -644,7 +896,10 @@ hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, c
                        const double* chunk_x, hipStream_t s) {
   if (nwork <= 0) return hipSuccess;
   if (bn == 64) return launch_cfg<1, 1>(prec, epi, work, nwork, probs, chunk_x, s);
-  if (bn == 128) return launch_cfg<2, 2>(prec, epi, work, nwork, probs, chunk_x, s);
+  if (bn == 128) {
+    if (prec == PREC_BF16X3) return launch_ws(epi, work, nwork, probs, chunk_x, s);
+    return launch_cfg<2, 2>(prec, epi, work, nwork, probs, chunk_x, s);
+  }
   return hipErrorInvalidValue;
 }
 
