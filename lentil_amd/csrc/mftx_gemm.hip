@@ -699,13 +699,24 @@ __device__ __forceinline__ void ws_a_set(WsACursor& ac, const Problem& P, long l
   for (int j = 0; j < 4; ++j) ac.unit[j] = 16 * (pu ^ (4 * j + rr));
 }
 
-// Direct-to-LDS loads of this wave's 16 rows of the cursor's chunk into A buffer ab.
+// Direct-to-LDS load j (rows 16 w + 4 j .. + 3) of the cursor's chunk into A
+// buffer ab: lane l's 16 bytes land at M0 + 16 l.  Written out rather than
+// through the builtin: the compiler counts a builtin LDS DMA on the LDS/scalar
+// counter too, and then waits for every fragment read at once (lgkmcnt(0))
+// where a count would do; the load has no register result, and the waits for
+// it (vmcnt) are explicit.  SALU writes of M0 need one wait state before an
+// LDS DMA reads it.
+__device__ __forceinline__ void ws_a_load1(ws::Lds& L, int ab, int w, const WsACursor& ac, int j) {
+  const char* src = ac.base + (long long)ac.c * 256 + j * ac.row4 + ac.unit[j];
+  const unsigned dst = (unsigned)(size_t)(ws::lds_void*)&L.a[ab][16 * w + 4 * j][0];
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :: "v"(src), "s"(__builtin_amdgcn_readfirstlane(dst)) : "memory", "m0");
+}
+
+// All 4 loads of this wave's 16 rows of the cursor's chunk into A buffer ab.
 __device__ __forceinline__ void ws_a_load(ws::Lds& L, int ab, int w, const WsACursor& ac) {
-  const char* src = ac.base + (long long)ac.c * 256;
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    __builtin_amdgcn_global_load_lds((ws::gvoid*)(src + j * ac.row4 + ac.unit[j]),
-                                     (ws::lds_void*)&L.a[ab][16 * w + 4 * j][0], 16, 0, 0);
+  for (int j = 0; j < 4; ++j) ws_a_load1(L, ab, w, ac, j);
 }
 
 // The producer's twiddle state for one problem: 8 consecutive K values of one
@@ -816,24 +827,63 @@ __device__ __forceinline__ void gemm_ws_body(const Work& wk, const Problem* __re
   int pc = wk.pbeg, cs = 0, cn = probs[pc].kpad / ws::BKC;
   int ab = 0, an = 2;  // A buffers of chunks s and s + 2
   for (int s = 0; s < nchunks; ++s) {
-    ws_a_load(L, an, wid, ac);
+    // Nothing is outstanding on the LDS/scalar counter here (the barrier
+    // wait above drained it); saying so with a compiler-visible wait lets the
+    // fragment reads below be waited for one by one instead of all at once.
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt, expcnt unrestricted
+    if (mf) {
+      // Operands of a 16-wide K block: A planes hi (re, im) and lo, twiddle
+      // planes hi (cos, sin) and lo.  The block's three products run in the
+      // order hi*lo, hi*hi, lo*hi, so each operand pair is dead as early as
+      // possible and the next block's fragment reads go into its registers
+      // right after the MFMAs that last read it: block 1's reads are in
+      // flight while block 0 multiplies, without extra registers.  The 4
+      // direct-to-LDS loads of chunk s + 2 sit between product groups, so
+      // the address unit works through the workgroup's 32 loads while the
+      // MFMAs issue.  Scheduling barriers pin this order.
+      const int ua = (lane >> 5) ^ (r0 & 15);  // A unit of (block 0, plane 0); block j, plane p: ua ^ (8 j + 2 p)
+      auto rd_a = [&](int j, int p) { return *(const bf16x8_t*)&L.a[ab][r0][8 * (ua ^ (8 * j + 2 * p))]; };
+      auto rd_b = [&](int j, int p) {
+        return *(const bf16x8_t*)&L.b[s & 1][p][c0][swz<2>(lane & 31, 2 * j + (lane >> 5))];
+      };
+      auto mf4 = [&](const bf16x8_t (&x)[2], const bf16x8_t (&y)[2]) {
+        acc[0] = MFMA_BF16(x[0], y[0], acc[0]);   // re cos
+        acc[1] = MFMA_BF16(x[1], y[0], acc[1]);   // im cos
+        acc[2] = MFMA_BF16(x[0], y[1], acc[2]);   // re sin
+        acc[3] = MFMA_BF16(x[1], y[1], acc[3]);   // im sin
+      };
+      auto load = [&](int j) {
+        __builtin_amdgcn_sched_barrier(0);
+        ws_a_load1(L, an, wid, ac, j);
+        __builtin_amdgcn_sched_barrier(0);
+      };
+      bf16x8_t ah[2] = {rd_a(0, 0), rd_a(0, 2)}, al[2] = {rd_a(0, 1), rd_a(0, 3)};
+      bf16x8_t bh[2] = {rd_b(0, 0), rd_b(0, 2)}, bl[2] = {rd_b(0, 1), rd_b(0, 3)};
+      mf4(ah, bl);
+      load(0);
+      bl[0] = rd_b(1, 1);
+      bl[1] = rd_b(1, 3);
+      mf4(ah, bh);
+      load(1);
+      ah[0] = rd_a(1, 0);
+      ah[1] = rd_a(1, 2);
+      mf4(al, bh);
+      __builtin_amdgcn_sched_barrier(0);
+      bh[0] = rd_b(1, 0);
+      bh[1] = rd_b(1, 2);
+      al[0] = rd_a(1, 1);
+      al[1] = rd_a(1, 3);
+      __builtin_amdgcn_sched_barrier(0);
+      mf4(ah, bl);
+      load(2);
+      mf4(ah, bh);
+      load(3);
+      mf4(al, bh);
+    } else {
+      ws_a_load(L, an, wid, ac);
+    }
     a_next();
     an = an == 2 ? 0 : an + 1;
-    if (mf) {
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int koff = swz<2>(lane & 31, 2 * j + (lane >> 5));
-        bf16x8_t a[4], bq[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          // logical unit of (K block j, plane p, half lane >> 5) in the row-major A buffer
-          const int u = (8 * j + 2 * p + (lane >> 5)) ^ (r0 & 15);
-          a[p] = *(const bf16x8_t*)&L.a[ab][r0][8 * u];
-          bq[p] = *(const bf16x8_t*)&L.b[s & 1][p][c0][koff];
-        }
-        sblock_bf16(a, bq, acc);
-      }
-    }
     asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
     ab = ab == 2 ? 0 : ab + 1;
     if (++cs == cn) {
