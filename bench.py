@@ -49,6 +49,8 @@ def parse():
     ap.add_argument('--contraction', default='auto', choices=['auto', 'per_tile', 'merged'],
                     help='segmented workloads: segment tiles one by one, summed into their union box, or '
                          'whichever executes fewer MACs')
+    ap.add_argument('--eager', action='store_true',
+                    help='launch the plan straight onto the stream each step instead of replaying a hipGraph')
     ap.add_argument('--copy-mib', type=int, default=0,
                     help='after the timed steps, run one plain device copy of this many MiB (a known-size '
                          'kernel that scales hardware byte counters)')
@@ -56,6 +58,27 @@ def parse():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'broadband_512_stage2_hbm_traffic.md')
+SETTLE_S = 0.2   # minimum seconds of back-to-back execution before the timed steps
+
+
+def warm_up(step, warmup, sync):
+    """Run ``step`` untimed at least ``warmup`` times and for at least
+    SETTLE_S seconds of back-to-back execution; returns the count.  The GPU's
+    power management answers a burst of load after idle with a clock
+    transient (a boost, then for some milliseconds a dip below the steady
+    state: stage-2 launches measured 368 us, then 460-480 us, then settling at
+    365-380 us), so a short timed window right after idle measures the
+    transient instead of the steady state.  The host waits for the device
+    every 8 steps so the clock counts execution, not queued launches."""
+    n = 0
+    t0 = time.perf_counter()
+    while n < warmup or time.perf_counter() - t0 < SETTLE_S:
+        step()
+        n += 1
+        if n % 8 == 0:
+            sync()
+    sync()
+    return n
 
 
 def measured_traffic(precision, lambda_group, tile_n):
@@ -128,9 +151,7 @@ def bench_segmented(args, name):
                          autotune=autotune, contraction=contraction)
 
     def timed(plan, out, steps, warmup):
-        for _ in range(warmup):
-            plan(out)
-        torch.cuda.synchronize()
+        warm_up(lambda: plan(out), warmup, torch.cuda.synchronize)
         t0 = time.perf_counter()
         for _ in range(steps):
             plan(out)
@@ -140,7 +161,9 @@ def bench_segmented(args, name):
     plan = make(args.contraction, tuned)
     out = torch.empty(plan.shape_out, dtype=torch.float32, device='cuda')
     dt = timed(plan, out, args.steps, args.warmup)
-    ms = plan.profile(torch.empty_like(out), repeats=3)
+    scratch = torch.empty_like(out)
+    warm_up(lambda: plan(scratch), 0, torch.cuda.synchronize)
+    ms = plan.profile(scratch, repeats=3)
     print(f'[bench] {name}: {1e3 * dt:.3f} ms/image, merged={plan.merged}', file=sys.stderr, flush=True)
     per_tile = None
     if plan.merged:
@@ -211,9 +234,7 @@ def bench_retrieval(args):
         loss = torch.sum((model(x) - target) ** 2)
         loss.backward()
         return loss
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    warmed = warm_up(step, args.warmup, torch.cuda.synchronize)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -229,6 +250,7 @@ def bench_retrieval(args):
                    'wavelengths': len(cfg['wavelengths']), 'pupil': 512, 'detector': 512,
                    'parallelism': 'single GPU'},
         'forward_psfs_per_s': npsf / dt,
+        'warmup_executed': warmed,
     }
     print(json.dumps(line))
 
@@ -254,11 +276,9 @@ def main():
     tuned = args.lambda_group == 0 and args.tile_n == 0
     plan = L.PSFPlan(pupil, cfg['wavelengths'], cfg['weights'], pixelscale=cfg['pixelscale'], shape=cfg['shape'],
                      oversample=cfg['oversample'], precision=args.precision, lambda_group=args.lambda_group or 8,
-                     tile_n=args.tile_n, use_graph=True, autotune=tuned)
+                     tile_n=args.tile_n, use_graph=not args.eager, autotune=tuned)
     out = torch.empty(plan.shape_out, dtype=torch.float32, device='cuda')
-    for _ in range(args.warmup):
-        plan(out)
-    torch.cuda.synchronize()
+    warmed = warm_up(lambda: plan(out), args.warmup, torch.cuda.synchronize)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan(out)
@@ -281,7 +301,9 @@ def main():
 
     # dominant kernel: stage-2 GEMM, HIP events on the plan's stream, timed in
     # the mode the steps ran in (graph replay, repeats back to back)
-    stage_ms = plan.profile(torch.empty_like(out), repeats=10)
+    scratch = torch.empty_like(out)
+    warm_up(lambda: plan(scratch), 0, torch.cuda.synchronize)
+    stage_ms = plan.profile(scratch, repeats=10)
     stage_sum = sum(stage_ms[k] for k in ('phasor', 'stage1', 'stage2', 'reduce'))
     flops2 = plan.info['gemm_flops'][1]
     ach = flops2 / (stage_ms['stage2'] * 1e-3)
@@ -342,8 +364,7 @@ def main():
                         oversample=cfg['oversample'], precision='fp32', lambda_group=plan.lambda_group,
                         tile_n=plan.tile_n)
         o32 = torch.empty_like(out)
-        p32(o32)
-        torch.cuda.synchronize()
+        warm_up(lambda: p32(o32), 1, torch.cuda.synchronize)
         t0 = time.perf_counter()
         for _ in range(3):
             p32(o32)
@@ -373,7 +394,9 @@ def main():
         'dtype': args.precision, 'data': 'synthetic (seeded pupil, Zernike OPD)',
         'config': {'workload': 'broadband_512', 'pupil': 512, 'detector': 1024, 'wavelengths': nwave,
                    'psfs_per_step': nwave, 'precision': args.precision, 'lambda_group': plan.lambda_group,
-                   'tile_n': plan.tile_n, 'autotuned': tuned, 'parallelism': 'single GPU'},
+                   'tile_n': plan.tile_n, 'autotuned': tuned, 'launch': 'eager' if args.eager else 'graph replay',
+                   'parallelism': 'single GPU'},
+        'warmup_executed': warmed,
         'tflops_algorithmic': total_flops * args.steps / dt / 1e12,
         'stage_ms': stage_ms,
         'roofline': roofline,

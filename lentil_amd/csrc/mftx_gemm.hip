@@ -705,13 +705,19 @@ __device__ __forceinline__ void ws_a_set(WsACursor& ac, const Problem& P, long l
 // counter too, and then waits for every fragment read at once (lgkmcnt(0))
 // where a count would do; the load has no register result, and the waits for
 // it (vmcnt) are explicit.  SALU writes of M0 need one wait state before an
-// LDS DMA reads it.
+// LDS DMA reads it.  The compiler reserves M0 and warns that it will not
+// restore it across the statement; these loads are the only M0 readers and
+// writers in the GEMM kernels (nothing else here uses M0), so nothing needs it
+// restored.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void ws_a_load1(ws::Lds& L, int ab, int w, const WsACursor& ac, int j) {
   const char* src = ac.base + (long long)ac.c * 256 + j * ac.row4 + ac.unit[j];
   const unsigned dst = (unsigned)(size_t)(ws::lds_void*)&L.a[ab][16 * w + 4 * j][0];
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
                :: "v"(src), "s"(__builtin_amdgcn_readfirstlane(dst)) : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 // All 4 loads of this wave's 16 rows of the cursor's chunk into A buffer ab.
 __device__ __forceinline__ void ws_a_load(ws::Lds& L, int ab, int w, const WsACursor& ac) {
