@@ -8,7 +8,8 @@ stage-2 GEMM with |E|^2 accumulated over wavelengths, detector reduction.
 Inputs are resident in HBM before the timed region.
 
 Printed: one JSON line with PSFs/s, the roofline of the dominant kernel
-(stage-2 GEMM, timed with HIP events on its stream), the relative-L2 error
+(stage-2 GEMM, timed as it runs in the replayed graph: stream events around
+back-to-back replays of prefix graphs), the relative-L2 error
 of the image against the complex128 reference loop, the plain per-wavelength
 FP32 GPU path and the CPU baseline (the complex128 oracle of the reference
 loop on this host's cores).
@@ -62,17 +63,20 @@ SETTLE_S = 0.2   # minimum seconds of back-to-back execution before the timed st
 
 
 def warm_up(step, warmup, sync):
-    """Run ``step`` untimed at least ``warmup`` times and for at least
-    SETTLE_S seconds of back-to-back execution; returns the count.  The GPU's
-    power management answers a burst of load after idle with a clock
-    transient (a boost, then for some milliseconds a dip below the steady
-    state: stage-2 launches measured 368 us, then 460-480 us, then settling at
-    365-380 us), so a short timed window right after idle measures the
-    transient instead of the steady state.  The host waits for the device
-    every 8 steps so the clock counts execution, not queued launches."""
-    n = 0
+    """Run ``step`` untimed ``warmup`` times (first calls may build plans),
+    then on for SETTLE_S seconds of back-to-back execution; returns the
+    count.  The GPU's power management answers a burst of load after idle
+    with a clock transient (in one kernel trace, stage-2 launches took
+    368 us, then 460-480 us, then settled near 410 us), so a short timed
+    window right after idle measures the transient instead of the steady
+    state.  The host waits for the device every 8 steps so the clock counts
+    execution, not queued launches."""
+    for _ in range(warmup):
+        step()
+    sync()
+    n = warmup
     t0 = time.perf_counter()
-    while n < warmup or time.perf_counter() - t0 < SETTLE_S:
+    while time.perf_counter() - t0 < SETTLE_S:
         step()
         n += 1
         if n % 8 == 0:
@@ -163,7 +167,7 @@ def bench_segmented(args, name):
     dt = timed(plan, out, args.steps, args.warmup)
     scratch = torch.empty_like(out)
     warm_up(lambda: plan(scratch), 0, torch.cuda.synchronize)
-    ms = plan.profile(scratch, repeats=3)
+    ms = plan.profile(scratch, repeats=10)
     print(f'[bench] {name}: {1e3 * dt:.3f} ms/image, merged={plan.merged}', file=sys.stderr, flush=True)
     per_tile = None
     if plan.merged:
@@ -299,11 +303,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst, evict
 
-    # dominant kernel: stage-2 GEMM, HIP events on the plan's stream, timed in
-    # the mode the steps ran in (graph replay, repeats back to back)
+    # per launch group in the mode the steps ran in (graph replay): differences
+    # of replayed prefix graphs, after the clocks have settled again
     scratch = torch.empty_like(out)
     warm_up(lambda: plan(scratch), 0, torch.cuda.synchronize)
-    stage_ms = plan.profile(scratch, repeats=10)
+    stage_ms = plan.profile(scratch, repeats=20)
     stage_sum = sum(stage_ms[k] for k in ('phasor', 'stage1', 'stage2', 'reduce'))
     flops2 = plan.info['gemm_flops'][1]
     ach = flops2 / (stage_ms['stage2'] * 1e-3)
@@ -370,7 +374,7 @@ def main():
             p32(o32)
         torch.cuda.synchronize()
         d32 = (time.perf_counter() - t0) / 3
-        ms32 = p32.profile(torch.empty_like(out), repeats=3)
+        ms32 = p32.profile(torch.empty_like(out), repeats=5)
         extra['batched_fp32'] = {'psfs_per_s': nwave / d32, 'stage2_tflops': p32.info['gemm_flops'][1] / ms32['stage2'] / 1e9,
                                  'stage_ms': ms32}
         fp32_img = o32.double().cpu().numpy()
@@ -394,7 +398,8 @@ def main():
         'dtype': args.precision, 'data': 'synthetic (seeded pupil, Zernike OPD)',
         'config': {'workload': 'broadband_512', 'pupil': 512, 'detector': 1024, 'wavelengths': nwave,
                    'psfs_per_step': nwave, 'precision': args.precision, 'lambda_group': plan.lambda_group,
-                   'tile_n': plan.tile_n, 'autotuned': tuned, 'launch': 'eager' if args.eager else 'graph replay',
+                   'tile_n': plan.tile_n, 'autotuned': tuned, 'autotune_source': plan.tuned_from,
+                   'launch': 'eager' if args.eager else 'graph replay',
                    'parallelism': 'single GPU'},
         'warmup_executed': warmed,
         'tflops_algorithmic': total_flops * args.steps / dt / 1e12,

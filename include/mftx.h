@@ -338,14 +338,18 @@ int mftx_plan_execute_fields(mftx_plan* plan);
  * on later executes with the same pointer (use_graph != 0), or run eagerly. */
 int mftx_plan_set_graph(mftx_plan* plan, int32_t use_graph);
 int mftx_plan_get_info(const mftx_plan* plan, mftx_plan_info* info);
-/* Run the plan's launch groups `repeats` times back to back (at most 4096)
- * with HIP events between them, in the mode executes run in: when the plan
- * replays a graph, all repeats are captured in one graph whose event-record
- * nodes time them (replayed once to warm up, then timed); otherwise they are
- * queued eagerly on the context stream.  ms[0..3] receive the mean
- * milliseconds of (phasors, stage-1 GEMM, stage-2 GEMM, detector reduction),
- * ms[4] is 1 when the graph mode was used and 0 when the launches ran eagerly
- * (also when the runtime refused event nodes in a graph).  The image is
+/* Time the plan's launch groups (phasors, stage-1 GEMM, stage-2 GEMM,
+ * detector reduction) in the mode executes run in, `repeats` times each (at
+ * most 4096).  When the plan replays a graph: one graph per prefix of the
+ * groups (the first k, k = 1..4) is replayed `repeats` times back to back
+ * between two stream events, after two full replays, the prefixes in the
+ * order 1 2 3 4 4 3 2 1 so a slow clock drift cancels; a group's time is the
+ * difference of the mean replay times of the prefixes that end with and
+ * before it, so each launch runs right after its producer with no event in
+ * between (the phasor time includes one graph launch's overhead).  Otherwise
+ * the groups are queued eagerly with events between them.  ms[0..3] receive
+ * the mean milliseconds per group, ms[4] is 1 for the graph mode and 0 for
+ * the eager one (also when the runtime refuses a capture).  The image is
  * overwritten.  Measurement hook; no reference counterpart. */
 int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi,
                       int32_t repeats, double* ms);

@@ -17,6 +17,9 @@ replayed from a hipGraph.  The per-Field geometry is the same host code as
 every shift, clip and offset.
 """
 import ctypes
+import hashlib
+import json
+import os
 from itertools import combinations
 
 import numpy as np
@@ -219,9 +222,61 @@ def plan_geometry(pupil, wavelengths, weights=None, pixelscale=None, shape=None,
 
 
 # (lambda_group, tile_n) chosen by PSFPlan(autotune=True), keyed by the
-# shapes that decide the schedule; lives for the process.
+# shapes that decide the schedule: kept for the process and in a JSON file
+# (LENTIL_AMD_TUNE_CACHE, default ~/.cache/lentil_amd/autotune.json; an empty
+# value turns the file off) whose entries also name the device and the built
+# library's hash, so another GPU model or a rebuilt library tunes afresh.
 _TUNED = {}
 _CANDIDATES = [(lg, tn) for tn in (64, 128) for lg in (4, 8, 16)]
+_LIB_HASH = None
+
+
+def tune_cache_path():
+    """Path of the persistent autotune cache, or None when it is turned off."""
+    path = os.environ.get('LENTIL_AMD_TUNE_CACHE')
+    if path is None:
+        path = os.path.join(os.path.expanduser('~'), '.cache', 'lentil_amd', 'autotune.json')
+    return path or None
+
+
+def _library_hash():
+    global _LIB_HASH
+    if _LIB_HASH is None:
+        with open(_lib.LIB_PATH, 'rb') as fh:
+            _LIB_HASH = hashlib.sha1(fh.read()).hexdigest()[:16]
+    return _LIB_HASH
+
+
+def tune_cache_key(key, device_name, lib_hash):
+    """The cache file's key for one schedule choice: a stable text form of the
+    shape key with the device name and the library hash."""
+    return json.dumps([device_name, lib_hash, key], separators=(',', ':'), default=int)
+
+
+def tune_cache_load(path):
+    """Entries of the cache file ({key: [lambda_group, tile_n]}); an absent or
+    unreadable file is an empty cache."""
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return {}
+    if not isinstance(data, dict):
+        return {}
+    return {k: (int(v[0]), int(v[1])) for k, v in data.items()
+            if isinstance(v, list) and len(v) == 2 and tuple(v) in _CANDIDATES}
+
+
+def tune_cache_store(path, ckey, choice):
+    """Add one entry, rewriting the file atomically (another process's
+    entries written in between may be lost; they are re-tuned on demand)."""
+    data = tune_cache_load(path)
+    data[ckey] = [int(choice[0]), int(choice[1])]
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    tmp = f'{path}.{os.getpid()}.tmp'
+    with open(tmp, 'w') as fh:
+        json.dump(data, fh, indent=0, sort_keys=True)
+    os.replace(tmp, path)
 
 
 class PSFPlan:
@@ -295,11 +350,25 @@ class PSFPlan:
         self._parr = (_lib.MftxProblem * max(len(problems), 1))(*problems)
         self._ntiles = len(tiles)
 
+        self.tuning = {}
+        self.tuned_from = None
         if autotune:
             key = (precision, self.shape_out, tuple(self.tile_shapes), len(self.wavelengths),
                    tuple((p['tile'], p['M'], p['N'], p['group']) for p in self.problems))
+            self.tuned_from = 'process'
             if key not in _TUNED:
-                _TUNED[key] = self._autotune()
+                path = tune_cache_path()
+                ckey = tune_cache_key(key, torch.cuda.get_device_name(dev), _library_hash()) if path else None
+                cached = tune_cache_load(path).get(ckey) if path else None
+                if cached is not None:
+                    _TUNED[key], self.tuned_from = cached, 'file'
+                else:
+                    _TUNED[key], self.tuned_from = self._autotune(), 'timed'
+                    if path:
+                        try:
+                            tune_cache_store(path, ckey, _TUNED[key])
+                        except OSError:
+                            pass   # a read-only home: the choice still holds for this process
             lambda_group, tile_n = _TUNED[key]
         self.lambda_group, self.tile_n = int(lambda_group), int(tile_n)
         self._plan = self._build(self.lambda_group, self.tile_n)
@@ -363,10 +432,12 @@ class PSFPlan:
         return out
 
     def profile(self, out=None, repeats=5):
-        """Mean milliseconds of (phasors, stage-1 GEMM, stage-2 GEMM, reduction),
-        timed with HIP events on the launch stream over ``repeats`` back-to-back
-        executes in the mode calls run in (graph replay when ``use_graph``);
-        ``mode`` says which mode the events timed."""
+        """Mean milliseconds of (phasors, stage-1 GEMM, stage-2 GEMM, reduction)
+        in the mode calls run in: with ``use_graph``, differences of the mean
+        replay times of prefix graphs (the first k launch groups, ``repeats``
+        back-to-back replays each between two stream events), so each group
+        runs right after its producer as in a replayed call; otherwise HIP
+        events between eagerly queued groups.  ``mode`` says which was used."""
         if out is None:
             out = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
         ms = (ctypes.c_double * 5)()

@@ -1861,22 +1861,76 @@ int mftx_plan_execute(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
 
 // One execute's launch groups with a timing event before, between and after
 // them (ev[0..4]) on stream s.
-static hipError_t profile_launches(mftx_plan* plan, void* image, int dtype, int64_t ldi, hipStream_t s,
+// The plan's launch groups in execute order (phasors, stage-1 GEMM, stage-2
+// GEMM, reduction or fallback), the first `upto` of them; with `ev`, an event
+// is recorded before the first and after each.
+static hipError_t profile_launches(mftx_plan* plan, void* image, int dtype, int64_t ldi, hipStream_t s, int upto,
                                    const hipEvent_t* ev) {
-  hipError_t e = hipEventRecord(ev[0], s);
-  if (e == hipSuccess) e = launch_phasors(plan, s);
-  if (e == hipSuccess) e = hipEventRecord(ev[1], s);
-  if (e == hipSuccess)
-    e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->bn, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s);
-  if (e == hipSuccess) e = hipEventRecord(ev[2], s);
-  if (e == hipSuccess)
-    e = launch_gemm(plan->prec, plan->epi2, plan->bn, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s);
-  if (e == hipSuccess) e = hipEventRecord(ev[3], s);
-  if (e == hipSuccess)
-    e = plan->epi2 == EPI_INTENSITY ? launch_reduces(plan, image, dtype, ldi, 0, s)
-                                    : launch_fallbacks(plan, image, dtype, ldi, 0, s);
-  if (e == hipSuccess) e = hipEventRecord(ev[4], s);
+  hipError_t e = ev ? hipEventRecord(ev[0], s) : hipSuccess;
+  for (int k = 0; k < upto && e == hipSuccess; ++k) {
+    if (k == 0) e = launch_phasors(plan, s);
+    if (k == 1)
+      e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->bn, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x,
+                      s);
+    if (k == 2)
+      e = launch_gemm(plan->prec, plan->epi2, plan->bn, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s);
+    if (k == 3)
+      e = plan->epi2 == EPI_INTENSITY ? launch_reduces(plan, image, dtype, ldi, 0, s)
+                                      : launch_fallbacks(plan, image, dtype, ldi, 0, s);
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[k + 1], s);
+  }
   return e;
+}
+
+// Graph mode of mftx_plan_profile: one graph per prefix of the launch groups
+// (the first k of them, k = 1..4), each replayed `repeats` times back to back
+// between two stream events, twice; group k's time is the difference of the
+// mean replay times of prefixes k + 1 and k, so every launch runs as in a
+// replayed execute, after its producer, with no event between launches.
+// Returns the runtime's error when it cannot capture.
+static hipError_t profile_graphs(mftx_plan* plan, void* image, int dtype, int64_t ldi, int repeats, double* ms) {
+  hipStream_t s = plan->ctx->stream;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t x[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+    hipGraph_t g = nullptr;
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t l = profile_launches(plan, image, dtype, ldi, cap, k + 1, nullptr);
+      e = hipStreamEndCapture(cap, &g);
+      if (e == hipSuccess) e = l;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&x[k], g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  // two full replays first: caches and clocks as in a run of executes
+  for (int r = 0; r < 2 && e == hipSuccess; ++r) e = hipGraphLaunch(x[3], s);
+  // prefixes timed in the order 1 2 3 4 4 3 2 1: a slow drift of the clock
+  // over the measurement adds the same to both timings of every prefix
+  double t[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 8 && e == hipSuccess; ++i) {
+    const int k = i < 4 ? i : 7 - i;
+    e = hipEventRecord(a, s);
+    for (int r = 0; r < repeats && e == hipSuccess; ++r) e = hipGraphLaunch(x[k], s);
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float f = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, a, b);
+    t[k] += 0.5 * (double)f / repeats;
+  }
+  for (auto& g : x)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (cap) (void)hipStreamDestroy(cap);
+  if (e != hipSuccess) return e;
+  ms[0] = t[0];
+  for (int k = 1; k < 4; ++k) ms[k] = t[k] - t[k - 1];
+  return hipSuccess;
 }
 
 int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t ldi, int32_t repeats, double* ms) {
@@ -1887,49 +1941,25 @@ int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
   if (ldi < plan->img_cols) return fail(ctx, MFTX_EINVAL, "image row stride too small");
   return guarded(ctx, [&]() -> int {
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (plan->use_graph) {
+    double g[4];
+    if (profile_graphs(plan, image, image_dtype, ldi, repeats, g) == hipSuccess) {
+      for (int k = 0; k < 4; ++k) ms[k] = g[k];
+      ms[4] = 1.0;
+      return MFTX_OK;
+    }
+    (void)hipGetLastError();  // the runtime refused a capture: time eagerly instead
+  }
+  // eager: every repeat back to back on the context stream with events
+  // between the launch groups, one wait at the end
   hipStream_t s = ctx->stream;
   std::vector<hipEvent_t> ev(5 * (size_t)repeats, nullptr);
   hipError_t e = hipSuccess;
   for (auto& x : ev)
     if (e == hipSuccess) e = hipEventCreate(&x);
-  bool graph = false;
-  if (e == hipSuccess && plan->use_graph) {
-    // The mode executes run in: every repeat back to back in one captured
-    // graph, the timing events as its event-record nodes.  Launched twice
-    // (the first replay warms the caches and clocks), timed on the second.
-    hipStream_t cap = nullptr;
-    hipGraph_t g = nullptr;
-    hipGraphExec_t x = nullptr;
-    hipError_t c = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
-    if (c == hipSuccess) c = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-    if (c == hipSuccess) {
-      hipError_t l = hipSuccess;
-      for (int r = 0; r < repeats && l == hipSuccess; ++r)
-        l = profile_launches(plan, image, image_dtype, ldi, cap, &ev[5 * (size_t)r]);
-      c = hipStreamEndCapture(cap, &g);
-      if (c == hipSuccess) c = l;
-    }
-    if (c == hipSuccess) c = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    if (c == hipSuccess) c = hipGraphLaunch(x, s);
-    if (c == hipSuccess) c = hipGraphLaunch(x, s);
-    if (c == hipSuccess) c = hipStreamSynchronize(s);
-    if (x) (void)hipGraphExecDestroy(x);
-    if (g) (void)hipGraphDestroy(g);
-    if (cap) (void)hipStreamDestroy(cap);
-    // the event nodes must also be readable as timing events
-    for (int k = 0; k < 4 && c == hipSuccess; ++k) {
-      float t = 0;
-      c = hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-    }
-    graph = c == hipSuccess;
-    if (!graph) (void)hipGetLastError();  // the runtime refused timed events in a graph: time eagerly instead
-  }
-  if (e == hipSuccess && !graph) {
-    // eager: every repeat back to back on the context stream, one wait at the end
-    for (int r = 0; r < repeats && e == hipSuccess; ++r)
-      e = profile_launches(plan, image, image_dtype, ldi, s, &ev[5 * (size_t)r]);
-    if (e == hipSuccess) e = hipEventSynchronize(ev.back());
-  }
+  for (int r = 0; r < repeats && e == hipSuccess; ++r)
+    e = profile_launches(plan, image, image_dtype, ldi, s, 4, &ev[5 * (size_t)r]);
+  if (e == hipSuccess) e = hipEventSynchronize(ev.back());
   double acc[4] = {0, 0, 0, 0};
   for (int r = 0; r < repeats && e == hipSuccess; ++r)
     for (int k = 0; k < 4 && e == hipSuccess; ++k) {
@@ -1941,7 +1971,7 @@ int mftx_plan_profile(mftx_plan* plan, void* image, int32_t image_dtype, int64_t
     if (x) (void)hipEventDestroy(x);
   if (e != hipSuccess) return hipfail(ctx, e, "plan profile");
   for (int k = 0; k < 4; ++k) ms[k] = acc[k] / repeats;
-  ms[4] = graph ? 1.0 : 0.0;
+  ms[4] = 0.0;
   return MFTX_OK;
   });
 }

@@ -216,10 +216,10 @@ def test_plan_graph_replays(L):
 
 
 def test_plan_profile_modes(L):
-    """profile() times the launch groups back to back in the mode calls run
-    in (a graph of repeats for a graph plan when the runtime can time event
-    nodes in a graph, eager launches otherwise), and the image it leaves is
-    the plan's image."""
+    """profile() times the launch groups in the mode calls run in: a graph
+    plan by differences of replayed prefix graphs (each group right after
+    its producer), an eager plan with events between launches; the image it
+    leaves is the plan's image."""
     from lentil_amd import workloads
     cfg = workloads.mono_256()
     p = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'], focal_length=10)
@@ -228,10 +228,13 @@ def test_plan_profile_modes(L):
         plan = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, use_graph=use_graph, lambda_group=4)
         ref = plan().clone()
         out = torch.full_like(ref, -1.0)
-        ms = plan.profile(out, repeats=3)
+        ms = plan.profile(out, repeats=8)
         torch.cuda.synchronize()
-        assert ms['mode'] in (mode, 'eager')
-        assert all(ms[k] > 0 for k in ('phasor', 'stage1', 'stage2', 'reduce'))
+        assert ms['mode'] == mode
+        t = [ms[k] for k in ('phasor', 'stage1', 'stage2', 'reduce')]
+        assert all(np.isfinite(t)) and sum(t) > 0
+        if not use_graph:
+            assert all(x > 0 for x in t)   # differences of replay times may dip below zero for tiny groups
         assert torch.equal(out, ref)
         plan.close()
 
@@ -351,14 +354,19 @@ def test_plan_bad_tile_n(L):
         L.PSFPlan(p, wl, wt, pixelscale=5e-6, shape=40, oversample=2, tile_n=96)
 
 
-def test_autotune(L):
+def test_autotune(L, tmp_path, monkeypatch):
     """The autotuner times every candidate, keeps one of them, and the tuned
-    plan still matches the oracle."""
+    plan still matches the oracle; later plans of the same shapes take the
+    choice from the process table, or, in a new process, from the cache
+    file."""
     from lentil_amd import batched, workloads
+    monkeypatch.setenv('LENTIL_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    monkeypatch.setattr(batched, '_TUNED', {})
     cfg = workloads.mono_256()
     p = L.Pupil(amplitude=cfg['amp'], opd=cfg['opd'], pixelscale=cfg['dx'], focal_length=10)
     wl = np.linspace(600e-9, 700e-9, 8)
     plan = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, autotune=True)
+    assert plan.tuned_from == 'timed'
     assert set(plan.tuning) == set(batched._CANDIDATES)
     assert all(t > 0 for t in plan.tuning.values())
     assert (plan.lambda_group, plan.tile_n) == min(plan.tuning, key=plan.tuning.get)
@@ -367,8 +375,15 @@ def test_autotune(L):
     assert rel_l2(plan().double().cpu().numpy(), ref) < TOL_I
     # a second plan of the same shapes reuses the choice without timing again
     again = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, autotune=True)
-    assert not hasattr(again, 'tuning')
+    assert again.tuned_from == 'process' and again.tuning == {}
     assert (again.lambda_group, again.tile_n) == (plan.lambda_group, plan.tile_n)
+    # with an empty process table (a new process) the file supplies it
+    monkeypatch.setattr(batched, '_TUNED', {})
+    third = L.PSFPlan(p, wl, pixelscale=5e-6, shape=128, oversample=2, autotune=True)
+    assert third.tuned_from == 'file' and third.tuning == {}
+    assert (third.lambda_group, third.tile_n) == (plan.lambda_group, plan.tile_n)
+    for q in (plan, again, third):
+        q.close()
 
 
 def test_plan_copies(L):

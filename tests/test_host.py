@@ -345,3 +345,29 @@ def test_merged_contraction_is_exact():
     assert not plan_geometry(*args)['merged']
     with pytest.raises(ValueError):
         plan_geometry(*args, contraction='merged')
+
+
+def test_tune_cache_file(tmp_path, monkeypatch):
+    """The persistent autotune cache: entries round-trip through the JSON
+    file, keys carry the device and library identity, an unreadable file or
+    an entry that is not a candidate schedule reads as absent, and the
+    environment variable moves or turns off the file."""
+    from lentil_amd import batched
+    path = str(tmp_path / 'sub' / 'tune.json')
+    key = ('bf16x3', (1024, 1024), ((512, 512),), np.int64(64), ((0, np.int64(1024), 1024, 0),))
+    k1 = batched.tune_cache_key(key, 'gpu A', 'abc')
+    assert k1 != batched.tune_cache_key(key, 'gpu B', 'abc') != batched.tune_cache_key(key, 'gpu A', 'abd')
+    assert batched.tune_cache_load(path) == {}
+    batched.tune_cache_store(path, k1, (16, 128))
+    batched.tune_cache_store(path, 'other', (4, 64))
+    assert batched.tune_cache_load(path) == {k1: (16, 128), 'other': (4, 64)}
+    with open(path, 'w') as fh:
+        fh.write('{"x": [3, 7], "y": [8, 128], "z": "no"}')
+    assert batched.tune_cache_load(path) == {'y': (8, 128)}
+    with open(path, 'w') as fh:
+        fh.write('not json')
+    assert batched.tune_cache_load(path) == {}
+    monkeypatch.setenv('LENTIL_AMD_TUNE_CACHE', path)
+    assert batched.tune_cache_path() == path
+    monkeypatch.setenv('LENTIL_AMD_TUNE_CACHE', '')
+    assert batched.tune_cache_path() is None
