@@ -217,16 +217,31 @@ int mftx_fft_filter(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int
                     const void* kernel, int kernel_dtype, int64_t ldk, int normalize, void* out, int out_dtype,
                     int64_t ldo, int precision);
 
-/* util.rescale (util.py:262-349) with mode 'nearest': resample img to
- * out_rows x out_cols at the coordinates (i - out/2) / scale + in/2 with
- * spline order 0, 1 or 3 (scipy map_coordinates semantics: 12 samples of edge
- * padding and the cubic B-spline prefilter for order 3); the mask (img != 0
- * when mask is NULL) is resampled with order 1 and zeroed below its dtype's
- * eps; unitary scales by sum(img) / sum(resampled); the output is the product
- * with the mask. */
+/* Boundary modes of util.rescale's map_coordinates (util.py:340-342), named
+ * as scipy.ndimage names them; cval is 0. */
+#define MFTX_MODE_NEAREST 0
+#define MFTX_MODE_REFLECT 1
+#define MFTX_MODE_GRID_MIRROR 2
+#define MFTX_MODE_MIRROR 3
+#define MFTX_MODE_GRID_WRAP 4
+#define MFTX_MODE_WRAP 5
+#define MFTX_MODE_CONSTANT 6
+#define MFTX_MODE_GRID_CONSTANT 7
+
+/* util.rescale (util.py:262-349): resample img (float32/float64, or
+ * complex64/complex128 whose real and imaginary parts are resampled
+ * separately, util.py:339-342) to out_rows x out_cols at the coordinates
+ * (i - out/2) / scale + in/2 with spline order 0..5 and boundary mode
+ * MFTX_MODE_* (scipy map_coordinates semantics: for orders >= 2 the B-spline
+ * prefilter, after 12 samples of padding in modes nearest and grid-constant);
+ * the mask (img != 0 when mask is NULL; a mask is real) is resampled with
+ * order 1, mode nearest, and zeroed below its dtype's eps; unitary scales by
+ * sum(img) / sum(resampled) (complex sums for a complex image); the output
+ * is the product with the mask, real for a real image and complex64 /
+ * complex128 for a complex one. */
 int mftx_rescale(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_t cols, int64_t ld, const void* mask,
-                 int mask_dtype, int64_t ldm, double scale, int64_t out_rows, int64_t out_cols, int order, int unitary,
-                 void* out, int out_dtype, int64_t ldo);
+                 int mask_dtype, int64_t ldm, double scale, int64_t out_rows, int64_t out_cols, int order, int mode,
+                 int unitary, void* out, int out_dtype, int64_t ldo);
 
 /* ---------------------------------------------------------------------------
  * Modal OPD operations (device pointers; masks: nseg arrays mask_stride

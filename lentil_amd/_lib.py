@@ -87,7 +87,7 @@ SIGNATURES = {
     'mftx_fft_filter': (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_vp, c_int, c_i64, c_int, c_vp, c_int,
                                 c_i64, c_int]),
     'mftx_rescale': (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_vp, c_int, c_i64, c_dbl, c_i64, c_i64,
-                             c_int, c_int, c_vp, c_int, c_i64]),
+                             c_int, c_int, c_int, c_vp, c_int, c_i64]),
     'mftx_ptt_fit': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, ctypes.c_int32, c_i64, c_i64,
                              c_dbl, c_dbl, c_vp]),
     'mftx_remove_tilt': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp, c_int, c_i64, c_i64, ctypes.c_int32,

@@ -1094,37 +1094,44 @@ int mftx_fft_filter(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int
 }
 
 int mftx_rescale(mftx_ctx* ctx, const void* img, int dtype, int64_t rows, int64_t cols, int64_t ld, const void* mask,
-                 int mask_dtype, int64_t ldm, double scale, int64_t out_rows, int64_t out_cols, int order, int unitary,
-                 void* out, int out_dtype, int64_t ldo) {
+                 int mask_dtype, int64_t ldm, double scale, int64_t out_rows, int64_t out_cols, int order, int mode,
+                 int unitary, void* out, int out_dtype, int64_t ldo) {
   if (!ctx) return MFTX_EHANDLE;
   if (!img || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
-  if ((dtype != 1 && dtype != 3) || (out_dtype != 1 && out_dtype != 3) || (mask && mask_dtype != 1 && mask_dtype != 3))
-    return fail(ctx, MFTX_EINVAL, "image, mask and output must be float32 or float64");
-  if (order != 0 && order != 1 && order != 3) return fail(ctx, MFTX_EINVAL, "spline order must be 0, 1 or 3");
+  const bool cplx = dtype == 0 || dtype == 2;
+  if (dtype < 0 || dtype > 3 || (mask && mask_dtype != 1 && mask_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "image must be float32, float64, complex64 or complex128 and a mask real");
+  if (cplx ? (out_dtype != 0 && out_dtype != 2) : (out_dtype != 1 && out_dtype != 3))
+    return fail(ctx, MFTX_EINVAL, "the output must be real for a real image and complex for a complex one");
+  if (order < 0 || order > 5) return fail(ctx, MFTX_EINVAL, "spline order must be in 0..5");
+  if (mode < MFTX_MODE_NEAREST || mode > MFTX_MODE_GRID_CONSTANT) return fail(ctx, MFTX_EINVAL, "unknown boundary mode");
   if (rows <= 0 || cols <= 0 || ld < cols || out_rows <= 0 || out_cols <= 0 || ldo < out_cols ||
       (mask && ldm < cols) || rows > (1 << 20) || cols > (1 << 20) || out_rows > (1 << 20) || out_cols > (1 << 20))
     return fail(ctx, MFTX_EINVAL, "bad rescale shape");
   if (!(scale > 0) || !std::isfinite(scale)) return fail(ctx, MFTX_EINVAL, "scale must be positive");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  const int npad = order > 1 ? 12 : 0;
+  const int npad = 12;  // the most any mode pads
   const int64_t np_ = (rows + 2 * npad) * (cols + 2 * npad), no = out_rows * out_cols;
-  const int64_t nb = (std::max(rows * cols, no) + 255) / 256;
+  const int64_t nb = (std::max(np_, no) + 255) / 256;
   const size_t b_pad = (size_t)round_up(np_ * 8, 256), b_out = (size_t)round_up(no * 8, 256);
-  int rc = ensure_aux(ctx, 2 * b_pad + 2 * b_out + sizeof(double) * (size_t)(nb + 2));
+  int rc = ensure_aux(ctx, 2 * b_pad + 3 * b_out + sizeof(double) * (size_t)(nb + 4));
   if (rc) return rc;
   char* a = (char*)ctx->aux;
   double* P = (double*)a;
   double* T = (double*)(a + b_pad);
   double* raw = (double*)(a + 2 * b_pad);
-  double* mval = (double*)(a + 2 * b_pad + b_out);
-  double* part = (double*)(a + 2 * b_pad + 2 * b_out);
+  double* raw_im = (double*)(a + 2 * b_pad + b_out);
+  double* mval = (double*)(a + 2 * b_pad + 2 * b_out);
+  double* part = (double*)(a + 2 * b_pad + 3 * b_out);
   double* sums = part + nb;
-  // the mask's dtype sets the threshold (util.py:339, numpy finfo eps)
+  // the mask's dtype sets the threshold (util.py:338-339, numpy finfo eps of
+  // the mask, or of the image's real dtype when the mask is derived from it)
   const int mdt = mask ? mask_dtype : dtype;
-  const double eps = mdt == 3 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+  const bool single = mdt == 0 || mdt == 1;
+  const double eps = single ? 1.1920928955078125e-07 : 2.220446049250313e-16;
   HIPCHK(ctx, launch_rescale(img, dtype, rows, cols, ld, mask ? mask : img, mask ? 2 : 1, mdt, mask ? ldm : ld, eps,
-                             scale, out_rows, out_cols, order, unitary, out, out_dtype, ldo, P, T, raw, mval, part,
-                             sums, ctx->stream));
+                             scale, out_rows, out_cols, order, mode, unitary, out, out_dtype, ldo, P, T, raw, raw_im,
+                             mval, part, sums, ctx->stream));
   return MFTX_OK;
 }
 

@@ -192,10 +192,23 @@ hipError_t launch_scale_ratio(void* x, int dtype, long long rows, long long cols
 // unitary scaling of rescale; P, T: (rows + 24) x (cols + 24) doubles; raw,
 // mval: orows x ocols doubles; part: one double per 256 elements of the
 // larger of the input and output; sums: 2 doubles
+// The B-spline prefilter of a spline order as its two-sided impulse response
+// h[|k|], |k| <= n (n = 0 for orders 0 and 1).
+constexpr int kPrefilterTaps = 48;
+struct PrefilterTaps {
+  int n;
+  double h[kPrefilterTaps + 1];
+};
+PrefilterTaps prefilter_taps(int order);
+// rescale (util.py:262-349): map_coordinates(order 0..5, mode MFTX_MODE_*)
+// of a real or complex (dtype 0 / 2) image, mask, unitary ratio; P and T hold
+// the padded image (rows + 24) x (cols + 24) doubles each, raw / raw_im / mval
+// the output size, part ceil(max(padded, output) / 256) doubles, sums 4.
 hipError_t launch_rescale(const void* img, int dtype, long long rows, long long cols, long long ld, const void* msrc,
                           int mask_mode, int mdt, long long ldm, double eps, double scale, long long orows,
-                          long long ocols, int order, int unitary, void* out, int odt, long long ldo, double* P,
-                          double* T, double* raw, double* mval, double* part, double* sums, hipStream_t s);
+                          long long ocols, int order, int mode, int unitary, void* out, int odt, long long ldo,
+                          double* P, double* T, double* raw, double* raw_im, double* mval, double* part, double* sums,
+                          hipStream_t s);
 
 // modal OPD operations (mftx_modal.hip)
 // out[seg][9] = normal equations of the piston/tip/tilt fit; part: nseg * ceil(rows*cols/256) * 9 doubles

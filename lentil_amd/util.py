@@ -102,29 +102,40 @@ def rebin(img, factor):
     return out
 
 
+# scipy.ndimage's boundary mode names and the device's codes (MFTX_MODE_*)
+_RESCALE_MODES = {'nearest': 0, 'reflect': 1, 'grid-mirror': 2, 'mirror': 3, 'grid-wrap': 4, 'wrap': 5,
+                  'constant': 6, 'grid-constant': 7}
+
+
 def rescale(img, scale, shape=None, mask=None, order=3, mode='nearest', unitary=True):
     """Rescale an image by spline interpolation (util.py:262-349) on the device.
 
     Follows scipy.ndimage.map_coordinates as the reference calls it: the
-    sample coordinates (i - out/2) / scale + in/2, the mask (nonzero pixels
-    when None) resampled with order 1 and zeroed below its dtype's eps, the
-    unitary weight correction, then the product with the mask.  Built for
-    mode 'nearest' (the default, and what pixelate uses) with orders 0, 1
-    and 3, on real images; other modes, orders and complex images raise
-    NotImplementedError.  NumPy in gives NumPy out of the image's float
-    dtype; a device tensor gives a device tensor.
+    sample coordinates (i - out/2) / scale + in/2, spline order 0-5 and any of
+    scipy's boundary modes (cval 0, as the reference never passes one), the
+    mask (nonzero pixels when None) resampled with order 1 and zeroed below
+    its dtype's eps, the unitary weight correction, then the product with the
+    mask.  A complex image has its real and imaginary parts resampled
+    separately and the complex ratio of the sums applied (util.py:339-345); it
+    gives a complex128 result.  NumPy in gives NumPy out (the image's float
+    dtype, or complex128); a device tensor gives a device tensor.
     """
     import ctypes
     import torch
     from . import _lib
     from ._arrays import is_dev, to_dev
-    if mode != 'nearest' or order not in (0, 1, 3):
-        raise NotImplementedError("the device rescale supports mode='nearest' with order 0, 1 or 3")
+    # scipy.ndimage's own errors for an order or mode it does not have
+    if int(order) != order or not 0 <= order <= 5:
+        raise RuntimeError('spline order not supported')
+    if mode not in _RESCALE_MODES:
+        raise RuntimeError('boundary mode not supported')
     host = not is_dev(img)
-    if (np.iscomplexobj(np.asarray(img)) if host else img.is_complex()):
-        raise NotImplementedError('the device rescale supports real images')
     t = to_dev(img)
-    if t.dtype not in (torch.float32, torch.float64):
+    cplx = t.is_complex()
+    if cplx:
+        if t.dtype not in (torch.complex64, torch.complex128):
+            t = t.to(torch.complex128)
+    elif t.dtype not in (torch.float32, torch.float64):
         t = t.to(torch.float64)
     t = t.contiguous()
     if t.dim() != 2:
@@ -139,18 +150,21 @@ def rescale(img, scale, shape=None, mask=None, order=3, mode='nearest', unitary=
     m = None
     if mask is not None:
         m = to_dev(mask)
+        if m.is_complex():
+            m = m.real
         if m.dtype not in (torch.float32, torch.float64):
             m = m.to(torch.float64)
         m = m.contiguous()
         if tuple(m.shape) != (rows, cols):
             raise ValueError(f'mask shape {tuple(m.shape)} != image shape {(rows, cols)}')
-    out = torch.empty((int(oshape[0]), int(oshape[1])), dtype=t.dtype, device=t.device)
+    out = torch.empty((int(oshape[0]), int(oshape[1])), dtype=torch.complex128 if cplx else t.dtype,
+                      device=t.device)
     lib = _lib.load()
     ctx = _lib.context()
     _lib.check(lib.mftx_rescale(ctx, ctypes.c_void_p(t.data_ptr()), _lib.dtype_code(t), rows, cols, cols,
                                 ctypes.c_void_p(m.data_ptr()) if m is not None else None,
                                 _lib.dtype_code(m) if m is not None else 0, cols, float(scale),
-                                out.shape[0], out.shape[1], int(order), 1 if unitary else 0,
+                                out.shape[0], out.shape[1], int(order), _RESCALE_MODES[mode], 1 if unitary else 0,
                                 ctypes.c_void_p(out.data_ptr()), _lib.dtype_code(out), out.shape[1]),
                ctx, 'mftx_rescale')
     return out.cpu().numpy() if host else out

@@ -3,7 +3,8 @@
 
 Tolerances: the FFT filters run two FP32 MFTs: rel-L2 <= 1e-5 (the workload
 gate; measured about 1e-6).  rebin and rescale compute in FP64: rebin equals
-the oracle to rounding, rescale within 1e-12 relative (float64 images) and
+the oracle to rounding, rescale within 1e-11 relative (float64 images, every
+boundary mode and spline order 0-5, real and complex) and
 FP32 rounding (float32 images).  The reference's own assertions
 (tests/test_detector.py:86-97, test_convolvable.py:5-36, test_util.py:13-36)
 are repeated on the device path.
@@ -132,5 +133,46 @@ def test_rescale_options(L):
     assert rel_l2(got.double().cpu().numpy(), ref.astype(float)) < 1e-6
     a = np.random.default_rng(0).uniform(low=0, high=1, size=(100, 100))     # tests/test_util.py:32-36
     assert np.isclose(np.sum(a), np.sum(L.rescale(a, scale=0.5, order=3, mode='nearest', unitary=True)))
-    with pytest.raises(NotImplementedError):
-        L.rescale(img, 0.5, mode='wrap')
+    # an order or mode scipy does not have raises scipy's RuntimeError
+    with pytest.raises(RuntimeError):
+        L.rescale(img, 0.5, order=6)
+    with pytest.raises(RuntimeError):
+        L.rescale(img, 0.5, mode='bogus')
+
+
+@pytest.mark.parametrize('order', range(6))
+@pytest.mark.parametrize('mode', ['nearest', 'reflect', 'grid-mirror', 'mirror', 'grid-wrap', 'wrap', 'constant',
+                                  'grid-constant'])
+def test_rescale_modes(L, mode, order):
+    """Every boundary mode and spline order against the oracle (scipy's
+    map_coordinates as util.rescale calls it), shrinking, growing, and with an
+    output larger than the image so coordinates fall outside it."""
+    rng = np.random.default_rng(17 * order + len(mode))
+    img = rng.random((37, 52))
+    img[:4] = 0
+    for kw in (dict(scale=0.5), dict(scale=1.7), dict(scale=1.0, shape=(60, 75))):
+        ref = D.rescale(img, order=order, mode=mode, **kw)
+        got = L.rescale(img, order=order, mode=mode, **kw)
+        assert got.shape == ref.shape and got.dtype == np.float64
+        assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max(), (mode, order, kw)
+
+
+@pytest.mark.parametrize('order,mode', [(3, 'nearest'), (1, 'constant'), (5, 'reflect'), (2, 'grid-wrap')])
+def test_rescale_complex(L, order, mode):
+    """Complex images: real and imaginary parts resampled separately, the
+    complex ratio of the sums applied, the mask taken from nonzero values
+    (util.py:328-347); complex128 out for complex128 and complex64 in."""
+    rng = np.random.default_rng(order)
+    img = rng.random((44, 39)) + 1j * rng.standard_normal((44, 39))
+    img[:3] = 0
+    img[10, :] = 1j * rng.random(39)          # zero real part, nonzero values
+    for kw in (dict(), dict(unitary=False), dict(shape=(50, 30))):
+        ref = D.rescale(img, 0.6, order=order, mode=mode, **kw)
+        got = L.rescale(img, 0.6, order=order, mode=mode, **kw)
+        assert got.dtype == np.complex128 and got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max(), kw
+    c64 = torch.tensor(img.astype(np.complex64), device='cuda')
+    got = L.rescale(c64, 0.6, order=order, mode=mode)
+    assert got.dtype == torch.complex128
+    ref = D.rescale(img.astype(np.complex64), 0.6, order=order, mode=mode)
+    assert rel_l2(got.cpu().numpy(), ref) < 1e-6

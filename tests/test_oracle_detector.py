@@ -111,17 +111,60 @@ def test_jitter_fwhm_known_answer():
 
 def test_host_errors_before_device_work():
     """The drop-in layer raises the reference's exception types before any
-    device work (util.py:243-244 for complex rebin data; this build's device
-    rescale covers mode 'nearest' and orders 0/1/3 only)."""
+    device work: util.py:243-244 for complex rebin data, and scipy's own
+    RuntimeError for a spline order or boundary mode map_coordinates does not
+    have (util.rescale passes both straight to it)."""
     from lentil_amd import util
     with pytest.raises(ValueError):
         util.rebin(np.ones((4, 4)) * 1j, 2)
-    with pytest.raises(NotImplementedError):
-        util.rescale(np.ones((4, 4)), 0.5, mode='wrap')
-    with pytest.raises(NotImplementedError):
-        util.rescale(np.ones((4, 4)), 0.5, order=2)
-    with pytest.raises(NotImplementedError):
-        util.rescale(np.ones((4, 4)) * 1j, 0.5)
+    with pytest.raises(RuntimeError, match='spline order'):
+        util.rescale(np.ones((4, 4)), 0.5, order=6)
+    with pytest.raises(RuntimeError, match='boundary mode'):
+        util.rescale(np.ones((4, 4)), 0.5, mode='bogus')
+    with pytest.raises(RuntimeError, match='spline order'):
+        nd.map_coordinates(np.ones((4, 4)), [[0.5], [0.5]], order=6)
+    with pytest.raises(RuntimeError, match='boundary mode'):
+        nd.map_coordinates(np.ones((4, 4)), [[0.5], [0.5]], mode='bogus')
     from lentil_amd import detector
     with pytest.raises(TypeError):
         detector.pixel(np.ones((4, 4)) * 1j)
+
+
+@pytest.mark.parametrize('order', range(6))
+@pytest.mark.parametrize('mode', D.MODES)
+def test_map_coordinates_explicit_matches_scipy(order, mode):
+    """The explicit restatement of map_coordinates (prefilter boundary rule,
+    padding, tap extension, 'constant' cut-off and 'wrap' coordinate map)
+    equals scipy's for every mode and spline order, at coordinates inside,
+    on and far outside the image, on an odd, non-square image.  Lines are at
+    least 24 samples: scipy starts its recursive prefilter from a truncated
+    boundary sum, which on shorter lines differs from the exact filter by
+    about |z|^(2n) (4e-8 for order 5 on 9 samples)."""
+    rng = np.random.default_rng(order * 11 + D.MODES.index(mode))
+    img = rng.standard_normal((25, 30))
+    ys = np.concatenate([np.linspace(-49.3, 77.1, 19), [0.0, 24.0, -0.5, 24.5, 3.5, -1e-12, -1.5, 25.5]])
+    xs = np.concatenate([np.linspace(-55.2, 89.6, 23), [0.0, 29.0, -0.5, 29.5, 6.5, 29 + 1e-12, -1.5, 30.5]])
+    xx, yy = np.meshgrid(xs, ys)
+    ref = nd.map_coordinates(img, [yy, xx], order=order, mode=mode)
+    got = D.map_coordinates_explicit(img, yy, xx, order, mode)
+    assert np.max(np.abs(got - ref)) < 1e-11 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_map_coordinates_explicit_planted_mistakes():
+    """Mistakes a restatement could make are caught by scipy: the reflect
+    and mirror rules swapped, and legacy 'wrap' taken as periodic."""
+    rng = np.random.default_rng(5)
+    img = rng.standard_normal((9, 14))
+    xx, yy = np.meshgrid(np.linspace(-3.3, 16.1, 17), np.linspace(-2.2, 11.4, 13))
+    saved = dict(D.TAP_RULE), dict(D.FILTER_RULE)
+    try:
+        D.TAP_RULE['reflect'] = 'mirror'
+        bad = D.map_coordinates_explicit(img, yy, xx, 3, 'reflect')
+        assert np.max(np.abs(bad - nd.map_coordinates(img, [yy, xx], order=3, mode='reflect'))) > 1e-3
+        D.TAP_RULE.update(saved[0])
+        D.FILTER_RULE['wrap'] = 'periodic'
+        bad = D.map_coordinates_explicit(img, yy, xx, 3, 'wrap')
+        assert np.max(np.abs(bad - nd.map_coordinates(img, [yy, xx], order=3, mode='wrap'))) > 1e-3
+    finally:
+        D.TAP_RULE.update(saved[0])
+        D.FILTER_RULE.update(saved[1])
