@@ -670,7 +670,9 @@ constexpr int NCONS = 8, NPROD = 4;
 constexpr int NT = 64 * (NCONS + NPROD);
 constexpr int BS = 64, BKC = 32;
 constexpr int NA = 3, NB = 2;
-struct Lds {
+// 16-byte alignment stated: the fragment reads are 16-byte LDS reads
+// (ds_read_b128), which the compiler emits only where it can prove it.
+struct alignas(16) Lds {
   __bf16 a[NA][BM][128];   // [row][16-byte unit ^ (row & 15)][8 K values]
   __bf16 b[NB][4][BS][32];
 };
