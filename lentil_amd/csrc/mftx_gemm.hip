@@ -659,7 +659,9 @@ __global__ __launch_bounds__((Cfg<WN, KS>::NT), (Cfg<WN, KS>::WG_PER_CU)) void g
 // chunk (two 16-wide K blocks of 4 planes) in 16-byte units swizzled as
 // unit ^ (row & 15): one load moves 4 whole rows (8 cache lines, every byte
 // used), and the fragment reads of 16 (or 8) consecutive rows fall on
-// distinct banks.  One barrier per chunk.  Consumers wait for their previous
+// distinct banks.  One barrier per chunk (in stage 2 before the chunk's last
+// product group, so the next chunk's first fragment reads run under that
+// group).  Consumers wait for their previous
 // chunk's loads (vmcnt(4): the 4 loads just issued may stay in flight; loads
 // complete in issue order, so epilogue stores in between do not weaken this)
 // and their fragment reads, producers for their twiddle stores.  The barrier
@@ -834,65 +836,103 @@ __device__ __forceinline__ void gemm_ws_body(const Work& wk, const Problem* __re
   asm volatile("s_waitcnt vmcnt(4)\n\ts_barrier" ::: "memory");
   int pc = wk.pbeg, cs = 0, cn = probs[pc].kpad / ws::BKC;
   int ab = 0, an = 2;  // A buffers of chunks s and s + 2
+  // Operands of a 16-wide K block: A planes hi (re, im) and lo, twiddle
+  // planes hi (cos, sin) and lo.
+  const int ua = (lane >> 5) ^ (r0 & 15);  // A unit of (block 0, plane 0); block j, plane p: ua ^ (8 j + 2 p)
+  auto rd_a = [&](int buf, int j, int p) { return *(const bf16x8_t*)&L.a[buf][r0][8 * (ua ^ (8 * j + 2 * p))]; };
+  auto rd_b = [&](int buf, int j, int p) {
+    return *(const bf16x8_t*)&L.b[buf][p][c0][swz<2>(lane & 31, 2 * j + (lane >> 5))];
+  };
+  auto mf4 = [&](const bf16x8_t (&x)[2], const bf16x8_t (&y)[2]) {
+    acc[0] = MFMA_BF16(x[0], y[0], acc[0]);   // re cos
+    acc[1] = MFMA_BF16(x[1], y[0], acc[1]);   // im cos
+    acc[2] = MFMA_BF16(x[0], y[1], acc[2]);   // re sin
+    acc[3] = MFMA_BF16(x[1], y[1], acc[3]);   // im sin
+  };
+  auto load = [&](int j) {
+    __builtin_amdgcn_sched_barrier(0);
+    ws_a_load1(L, an, wid, ac, j);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  // The intensity epilogue (stage 2) reads chunk s + 1's first fragments
+  // under chunk s's last product group, across the chunk's barrier.  The
+  // store epilogues need those 16 registers at problem ends (holding them
+  // triples stage 1's spills), so they read a chunk's fragments at its top.
+  constexpr bool kAhead = EPI == EPI_INTENSITY;
+  // Block 0's A hi and twiddle lo fragments of chunk s.
+  bf16x8_t ah[2] = {}, bl[2] = {};
+  // Nothing is outstanding on the LDS/scalar counter after this wait; the
+  // compiler sees it (the barrier's own waits are in asm it cannot read), so
+  // it waits for each fragment read below by count, not for all at once.
+  auto drained = [&]() { __builtin_amdgcn_s_waitcnt(0xc07f); };  // lgkmcnt(0); vmcnt, expcnt unrestricted
+  auto barrier = [&]() {
+    // Wait for this wave's chunk s + 1 loads (the 4 just issued for chunk
+    // s + 2 may stay in flight: loads complete in issue order, so epilogue
+    // stores in between do not weaken this) and its fragment reads.
+    asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    drained();
+  };
+  auto first_reads = [&](int abuf, int bbuf) {
+    ah[0] = rd_a(abuf, 0, 0);
+    ah[1] = rd_a(abuf, 0, 2);
+    bl[0] = rd_b(bbuf, 0, 1);
+    bl[1] = rd_b(bbuf, 0, 3);
+  };
+  drained();
+  if (kAhead && mf) first_reads(0, 0);
   for (int s = 0; s < nchunks; ++s) {
-    // Nothing is outstanding on the LDS/scalar counter here (the barrier
-    // wait above drained it); saying so with a compiler-visible wait lets the
-    // fragment reads below be waited for one by one instead of all at once.
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt, expcnt unrestricted
+    const int b0 = s & 1;
     if (mf) {
-      // Operands of a 16-wide K block: A planes hi (re, im) and lo, twiddle
-      // planes hi (cos, sin) and lo.  The block's three products run in the
-      // order hi*lo, hi*hi, lo*hi, so each operand pair is dead as early as
-      // possible and the next block's fragment reads go into its registers
-      // right after the MFMAs that last read it: block 1's reads are in
-      // flight while block 0 multiplies, without extra registers.  The 4
-      // direct-to-LDS loads of chunk s + 2 sit between product groups, so
-      // the address unit works through the workgroup's 32 loads while the
-      // MFMAs issue.  Scheduling barriers pin this order.
-      const int ua = (lane >> 5) ^ (r0 & 15);  // A unit of (block 0, plane 0); block j, plane p: ua ^ (8 j + 2 p)
-      auto rd_a = [&](int j, int p) { return *(const bf16x8_t*)&L.a[ab][r0][8 * (ua ^ (8 * j + 2 * p))]; };
-      auto rd_b = [&](int j, int p) {
-        return *(const bf16x8_t*)&L.b[s & 1][p][c0][swz<2>(lane & 31, 2 * j + (lane >> 5))];
-      };
-      auto mf4 = [&](const bf16x8_t (&x)[2], const bf16x8_t (&y)[2]) {
-        acc[0] = MFMA_BF16(x[0], y[0], acc[0]);   // re cos
-        acc[1] = MFMA_BF16(x[1], y[0], acc[1]);   // im cos
-        acc[2] = MFMA_BF16(x[0], y[1], acc[2]);   // re sin
-        acc[3] = MFMA_BF16(x[1], y[1], acc[3]);   // im sin
-      };
-      auto load = [&](int j) {
-        __builtin_amdgcn_sched_barrier(0);
-        ws_a_load1(L, an, wid, ac, j);
-        __builtin_amdgcn_sched_barrier(0);
-      };
-      bf16x8_t ah[2] = {rd_a(0, 0), rd_a(0, 2)}, al[2] = {rd_a(0, 1), rd_a(0, 3)};
-      bf16x8_t bh[2] = {rd_b(0, 0), rd_b(0, 2)}, bl[2] = {rd_b(0, 1), rd_b(0, 3)};
+      if (!kAhead) first_reads(ab, b0);
+      // The block's three products run in the order hi*lo, hi*hi, lo*hi, so
+      // each operand pair is dead as early as possible and the next
+      // fragments are read into its registers right after the MFMAs that
+      // last read it: block 1's reads are in flight while block 0
+      // multiplies, and the next chunk's first reads while the last group
+      // multiplies, without extra registers.  The 4 direct-to-LDS loads of
+      // chunk s + 2 sit between product groups, so the address unit works
+      // through the workgroup's 32 loads while the MFMAs issue.  Reading
+      // ahead, the chunk's barrier comes before its last group: by then
+      // every fragment read of chunk s and every load of chunk s + 2 is
+      // issued, and after it chunk s + 1 is complete in LDS.  Scheduling
+      // barriers pin this order.
+      bf16x8_t al[2] = {rd_a(ab, 0, 1), rd_a(ab, 0, 3)};
+      bf16x8_t bh[2] = {rd_b(b0, 0, 0), rd_b(b0, 0, 2)};
       mf4(ah, bl);
       load(0);
-      bl[0] = rd_b(1, 1);
-      bl[1] = rd_b(1, 3);
+      bl[0] = rd_b(b0, 1, 1);
+      bl[1] = rd_b(b0, 1, 3);
       mf4(ah, bh);
       load(1);
-      ah[0] = rd_a(1, 0);
-      ah[1] = rd_a(1, 2);
+      ah[0] = rd_a(ab, 1, 0);
+      ah[1] = rd_a(ab, 1, 2);
       mf4(al, bh);
       __builtin_amdgcn_sched_barrier(0);
-      bh[0] = rd_b(1, 0);
-      bh[1] = rd_b(1, 2);
-      al[0] = rd_a(1, 1);
-      al[1] = rd_a(1, 3);
+      bh[0] = rd_b(b0, 1, 0);
+      bh[1] = rd_b(b0, 1, 2);
+      al[0] = rd_a(ab, 1, 1);
+      al[1] = rd_a(ab, 1, 3);
       __builtin_amdgcn_sched_barrier(0);
       mf4(ah, bl);
       load(2);
       mf4(ah, bh);
       load(3);
+      if (kAhead) {
+        barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        // chunk s + 1's first fragments (past the last chunk: a buffer
+        // nobody writes any more, read and unused)
+        first_reads(ab == 2 ? 0 : ab + 1, b0 ^ 1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
       mf4(al, bh);
+      if (!kAhead) barrier();
     } else {
       ws_a_load(L, an, wid, ac);
+      barrier();
     }
     a_next();
     an = an == 2 ? 0 : an + 1;
-    asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
     ab = ab == 2 ? 0 : ab + 1;
     if (++cs == cn) {
       problem_epilogue<PREC_BF16X3, EPI>(probs[pc], wk, ws::BS, wr, wc, lane, acc, ip, im);
