@@ -266,6 +266,15 @@ __device__ __forceinline__ void sblock_bf16(const bf16x8_t (&a)[4], const bf16x8
   }
 }
 
+// A C row index as FP64, converted where it is used.  The row indices of the
+// epilogue do not change along the K loop, so the compiler would hoist their
+// conversions out of it and spill them; reloading a spill at a problem end
+// waits for every load in flight (the operand loads of the next chunks too).
+__device__ __forceinline__ double row_f64(int r) {
+  asm volatile("" : "+v"(r));
+  return (double)r;
+}
+
 // End of one problem.  Accumulator element e of a lane is C row
 // 8 (e / 4) + 4 (lane / 32) + e % 4 of its 32-row block and slot lane % 32;
 // the slot's two output columns are jp = cp + s (P + iQ) and jm = cm - s
@@ -368,7 +377,7 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float pr = acc[0][4 * g + e], pi = acc[1][4 * g + e], qr = acc[2][4 * g + e], qi = acc[3][4 * g + e];
-          const float2 f = cis_turns(P.fold * (P.fold_x0 + (double)(row0 + e)));
+          const float2 f = cis_turns(P.fold * (P.fold_x0 + row_f64(row0 + e)));
           const float ar = (pr - qi) * sc, ai = (pi + qr) * sc, br = (pr + qi) * sc, bi = (pi - qr) * sc;
           v[0][0][e] = ar * f.x - ai * f.y;
           v[0][1][e] = ar * f.y + ai * f.x;
@@ -445,7 +454,7 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
       // the next stage's fold along its K index (this C's row)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float2 f = cis_turns(P.fold * (P.fold_x0 + (double)(row0 + e)));
+        const float2 f = cis_turns(P.fold * (P.fold_x0 + row_f64(row0 + e)));
         float t = pre[e] * f.x - pim[e] * f.y;
         pim[e] = pre[e] * f.y + pim[e] * f.x;
         pre[e] = t;

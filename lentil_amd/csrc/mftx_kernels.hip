@@ -152,6 +152,22 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
   return dtype == 3 ? ((const double*)p)[idx] : (double)((const float*)p)[idx];
 }
 
+// Global-memory views: pointers that arrive inside descriptors are generic,
+// and generic (flat) accesses also count on the LDS/scalar counter, so every
+// wait for a scalar load would drain them too.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ PhasorWave load_wave(gptr<const PhasorWave> p) {
+  PhasorWave w;
+  w.inv_wl = p->inv_wl;
+  w.fold = p->fold;
+  w.x0 = p->x0;
+  return w;
+}
+
 // One thread per (op, row, half of a 16-wide K block, group of PH_WAVES
 // wavelengths), every phasor operand of a plan in one launch (the op is found
 // by a block-uniform search).  The 8 amp/opd values are read once and the
@@ -173,46 +189,84 @@ __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, 
   const long long row = op.trans ? r % op.rows_pad : r / khalves;
   const int kh = (int)(op.trans ? r / op.rows_pad : r % khalves);
   const int kb = kh >> 1, i0 = (kh & 1) * 8;
-  float a[8];
-  double o[8];
+  // Every load is issued before any math and none sits behind a branch, so
+  // they are in flight together: an element outside the m x n tile reads
+  // element 0 and is zeroed afterwards, the dtypes are tested once, and the
+  // thread's wavelength descriptors are read up front.
+  long long ia[8], io[8];
+  bool in[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const long long k = (long long)kb * 16 + i0 + i;
     const long long sr = op.trans ? k : row, sc = op.trans ? row : k;
-    const bool in = sr < op.m && sc < op.n;
-    a[i] = in ? (float)load_real(op.amp, op.amp_dtype, sr * op.ld_amp + sc) : 0.f;
-    o[i] = in ? load_real(op.opd, op.opd_dtype, sr * op.ld_opd + sc) : 0.0;
+    in[i] = sr < op.m && sc < op.n;
+    ia[i] = in[i] ? sr * op.ld_amp + sc : 0;
+    io[i] = in[i] ? sr * op.ld_opd + sc : 0;
+  }
+  float a[8];
+  double o[8];
+  if (op.amp_dtype == 3) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = (float)((gptr<const double>)op.amp)[ia[i]];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = ((gptr<const float>)op.amp)[ia[i]];
+  }
+  if (op.opd_dtype == 3) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o[i] = ((gptr<const double>)op.opd)[io[i]];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o[i] = (double)((gptr<const float>)op.opd)[io[i]];
+  }
+  const int w0 = g * PH_WAVES, nw = min(op.nwave - w0, PH_WAVES);
+  PhasorWave pws[PH_WAVES];
+#pragma unroll
+  for (int j = 0; j < PH_WAVES; ++j) pws[j] = load_wave((gptr<const PhasorWave>)op.waves + w0 + min(j, nw - 1));
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    a[i] = in[i] ? a[i] : 0.f;
+    o[i] = in[i] ? o[i] : 0.0;
   }
   const long long lda = packed_lda(prec, op.kpad);
-  const int w_end = min(op.nwave, (g + 1) * PH_WAVES);
-  for (int w = g * PH_WAVES; w < w_end; ++w) {
-    const PhasorWave pw = op.waves[w];
-    const double f0 = pw.fold * (pw.x0 + (double)kb * 16);
-    float2 v[8];
+  // The precision is tested once, outside the wavelength loop, so each
+  // loop is straight-line code whose stores are never waited for.
+  auto store_all = [&](auto bf16x3) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const float2 c = cis_turns(__builtin_fma(o[i], pw.inv_wl, __builtin_fma(pw.fold, (double)(i0 + i), f0)));
-      v[i] = make_float2(a[i] * c.x, a[i] * c.y);
-    }
-    char* dst = (char*)op.dst + (long long)w * op.wave_stride;
-    if (prec == PREC_BF16X3) {
-      uint32_t rh[4], rl[4], ih[4], il[4];
+    for (int j = 0; j < PH_WAVES; ++j) {
+      if (j >= nw) break;
+      const PhasorWave pw = pws[j];
+      const double f0 = pw.fold * (pw.x0 + (double)kb * 16);
+      float2 v[8];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        split2(v[2 * i].x, v[2 * i + 1].x, rh[i], rl[i]);
-        split2(v[2 * i].y, v[2 * i + 1].y, ih[i], il[i]);
+      for (int i = 0; i < 8; ++i) {
+        const float2 c = cis_turns(__builtin_fma(o[i], pw.inv_wl, __builtin_fma(pw.fold, (double)(i0 + i), f0)));
+        v[i] = make_float2(a[i] * c.x, a[i] * c.y);
       }
-      uint4* d = (uint4*)((__bf16*)dst + row * lda + (long long)kb * 64 + i0);  // plane p at + 16 p bf16
-      d[0] = make_uint4(rh[0], rh[1], rh[2], rh[3]);
-      d[2] = make_uint4(rl[0], rl[1], rl[2], rl[3]);
-      d[4] = make_uint4(ih[0], ih[1], ih[2], ih[3]);
-      d[6] = make_uint4(il[0], il[1], il[2], il[3]);
-    } else {
-      float4* d = (float4*)((float2*)dst + row * lda + (long long)kb * 16 + i0);
+      char* dst = (char*)op.dst + (long long)(w0 + j) * op.wave_stride;
+      if constexpr (decltype(bf16x3)::value) {
+        uint32_t rh[4], rl[4], ih[4], il[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) d[i] = make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+        for (int i = 0; i < 4; ++i) {
+          split2(v[2 * i].x, v[2 * i + 1].x, rh[i], rl[i]);
+          split2(v[2 * i].y, v[2 * i + 1].y, ih[i], il[i]);
+        }
+        gptr<v4u32> d = (gptr<v4u32>)((__bf16*)dst + row * lda + (long long)kb * 64 + i0);  // plane p at + 16 p bf16
+        d[0] = v4u32{rh[0], rh[1], rh[2], rh[3]};
+        d[2] = v4u32{rl[0], rl[1], rl[2], rl[3]};
+        d[4] = v4u32{ih[0], ih[1], ih[2], ih[3]};
+        d[6] = v4u32{il[0], il[1], il[2], il[3]};
+      } else {
+        gptr<v4f32> d = (gptr<v4f32>)((float2*)dst + row * lda + (long long)kb * 16 + i0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = v4f32{v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y};
+      }
     }
-  }
+  };
+  if (prec == PREC_BF16X3)
+    store_all(std::true_type{});
+  else
+    store_all(std::false_type{});
 }
 
 hipError_t launch_phasor_group(int prec, const PhasorOp* ops, int nops, long long total, hipStream_t s) {
