@@ -7,6 +7,16 @@ namespace mftx {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
+typedef float v2f32 __attribute__((ext_vector_type(2)));
+
+// Global-memory views: pointers that arrive inside descriptors are generic,
+// and generic (flat) accesses also count on the LDS/scalar counter, so every
+// wait for a scalar load would drain them too.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
 
 // --------------------------------------------------------------------------
 // scalar helpers

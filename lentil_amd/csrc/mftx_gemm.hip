@@ -275,6 +275,22 @@ __device__ __forceinline__ double row_f64(int r) {
   return (double)r;
 }
 
+// Weight-map values at map offsets idx[0..n-1] (1 without a map).  No load
+// sits behind a branch, so they are in flight together.
+template <int NW>
+__device__ __forceinline__ void load_weights(const Problem& P, const long long (&idx)[NW], float (&wv)[NW]) {
+  if (!P.wmap) {
+#pragma unroll
+    for (int t = 0; t < NW; ++t) wv[t] = 1.0f;
+  } else if (P.wdtype == 3) {
+#pragma unroll
+    for (int t = 0; t < NW; ++t) wv[t] = (float)((gptr<const double>)P.wmap)[idx[t]];
+  } else {
+#pragma unroll
+    for (int t = 0; t < NW; ++t) wv[t] = ((gptr<const float>)P.wmap)[idx[t]];
+  }
+}
+
 // End of one problem.  Accumulator element e of a lane is C row
 // 8 (e / 4) + 4 (lane / 32) + e % 4 of its 32-row block and slot lane % 32;
 // the slot's two output columns are jp = cp + s (P + iQ) and jm = cm - s
@@ -301,6 +317,11 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
   }
   const float sc = P.scale;
   const bool has_p = jp < N, has_m = jm >= 0 && jm < N && jm != jp;
+  // the first C row of this wave's block, opaque to the compiler: the
+  // indices below then stay in the epilogue instead of being hoisted out of
+  // the K loop and spilled
+  int rblk = wk.tr * BM + wr * 32;
+  asm volatile("" : "+v"(rblk));
   if (EPI == EPI_STORE_CPLX && P.trans) {
     // Transposed complex store: C row r of output column j goes to
     // dst[j * ldd + r].  As for the packed store, one lane swap per word
@@ -322,7 +343,7 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
           v[1][gg][2 * e + 1] = (pi - qr) * sc;
         }
       }
-      const int r0 = wk.tr * BM + wr * 32 + 16 * h + 8 * (lane >> 5);  // first of this lane's 8 rows
+      const int r0 = rblk + 16 * h + 8 * (lane >> 5);  // first of this lane's 8 rows
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
         float o[16];
@@ -335,24 +356,22 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
         }
         if (!(side ? has_m : has_p)) continue;
         const int j = side ? jm : jp;
-        float wv[8];
+        // rows past P.rows (not stored) read the column's row 0
+        long long wi[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const long long wi = (long long)j * P.ldw + r0 + t;
-          wv[t] = !P.wmap || r0 + t >= P.rows ? 1.0f
-                                              : (P.wdtype == 3 ? (float)((const double*)P.wmap)[wi]
-                                                               : ((const float*)P.wmap)[wi]);
-        }
+        for (int t = 0; t < 8; ++t) wi[t] = (long long)j * P.ldw + (r0 + t < P.rows ? r0 + t : 0);
+        float wv[8];
+        load_weights(P, wi, wv);
         float2* d = dst + (long long)j * P.ldd + r0;
         if (r0 + 8 <= P.rows && (((uintptr_t)d) & 15) == 0) {
 #pragma unroll
           for (int t = 0; t < 4; ++t)
-            ((float4*)d)[t] = make_float4(o[4 * t] * wv[2 * t], o[4 * t + 1] * wv[2 * t],
-                                          o[4 * t + 2] * wv[2 * t + 1], o[4 * t + 3] * wv[2 * t + 1]);
+            ((gptr<v4f32>)d)[t] = v4f32{o[4 * t] * wv[2 * t], o[4 * t + 1] * wv[2 * t], o[4 * t + 2] * wv[2 * t + 1],
+                                        o[4 * t + 3] * wv[2 * t + 1]};
         } else {
 #pragma unroll
           for (int t = 0; t < 8; ++t)
-            if (r0 + t < P.rows) d[t] = make_float2(o[2 * t] * wv[t], o[2 * t + 1] * wv[t]);
+            if (r0 + t < P.rows) ((gptr<v2f32>)d)[t] = v2f32{o[2 * t] * wv[t], o[2 * t + 1] * wv[t]};
         }
       }
     }
@@ -408,7 +427,7 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
           __bf16* d = (__bf16*)P.dst + (long long)(side ? jm : jp) * P.ldd + (long long)(kb >> 4) * 64 + 8 * (lane >> 5);
 #pragma unroll
           for (int pl = 0; pl < 4; ++pl)
-            *(uint4*)(d + 16 * pl) = make_uint4(o[pl][0], o[pl][1], o[pl][2], o[pl][3]);
+            *(gptr<v4u32>)(d + 16 * pl) = v4u32{o[pl][0], o[pl][1], o[pl][2], o[pl][3]};
         }
       }
     }
@@ -417,7 +436,7 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     // rows row0 .. row0+3 of C in this lane
-    const int row0 = wk.tr * BM + wr * 32 + 8 * g + 4 * (lane >> 5);
+    const int row0 = rblk + 8 * g + 4 * (lane >> 5);
     float pre[4], pim[4], mre[4], mim[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -429,24 +448,29 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
     }
     if (EPI == EPI_STORE_CPLX) {
       float2* dst = (float2*)P.dst;
-      // optional real weight map, indexed like the destination
-      auto weight = [&](long long idx) -> float {
-        if (!P.wmap) return 1.0f;
-        return P.wdtype == 3 ? (float)((const double*)P.wmap)[idx] : ((const float*)P.wmap)[idx];
-      };
+      // optional real weight map, indexed like the destination; an element
+      // that is not stored reads the map's first value
+      long long wi[8];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = row0 + e;
+        const bool in = row < P.rows;
+        wi[e] = in && has_p ? (P.trans ? (long long)jp * P.ldw + row : (long long)row * P.ldw + jp) : 0;
+        wi[4 + e] = in && has_m ? (P.trans ? (long long)jm * P.ldw + row : (long long)row * P.ldw + jm) : 0;
+      }
+      float wv[8];
+      load_weights(P, wi, wv);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = row0 + e;
         if (row < P.rows) {
           if (has_p) {
             const long long o = P.trans ? (long long)jp * P.ldd + row : (long long)row * P.ldd + jp;
-            const float w = weight(P.trans ? (long long)jp * P.ldw + row : (long long)row * P.ldw + jp);
-            dst[o] = make_float2(pre[e] * w, pim[e] * w);
+            ((gptr<v2f32>)dst)[o] = v2f32{pre[e] * wv[e], pim[e] * wv[e]};
           }
           if (has_m) {
             const long long o = P.trans ? (long long)jm * P.ldd + row : (long long)row * P.ldd + jm;
-            const float w = weight(P.trans ? (long long)jm * P.ldw + row : (long long)row * P.ldw + jm);
-            dst[o] = make_float2(mre[e] * w, mim[e] * w);
+            ((gptr<v2f32>)dst)[o] = v2f32{mre[e] * wv[4 + e], mim[e] * wv[4 + e]};
           }
         }
       }
@@ -475,14 +499,14 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
           split2(imv[0], imv[1], ih0, il0);
           split2(imv[2], imv[3], ih1, il1);
           __bf16* d = (__bf16*)P.dst + (long long)j * P.ldd + (long long)(row0 >> 4) * 64 + (row0 & 15);
-          *(uint2*)(d + 0) = make_uint2(rh0, rh1);
-          *(uint2*)(d + 16) = make_uint2(rl0, rl1);
-          *(uint2*)(d + 32) = make_uint2(ih0, ih1);
-          *(uint2*)(d + 48) = make_uint2(il0, il1);
+          *(gptr<v2u32>)(d + 0) = v2u32{rh0, rh1};
+          *(gptr<v2u32>)(d + 16) = v2u32{rl0, rl1};
+          *(gptr<v2u32>)(d + 32) = v2u32{ih0, ih1};
+          *(gptr<v2u32>)(d + 48) = v2u32{il0, il1};
         } else {
-          float4* d = (float4*)((float2*)P.dst + (long long)j * P.ldd + row0);
-          d[0] = make_float4(re[0], imv[0], re[1], imv[1]);
-          d[1] = make_float4(re[2], imv[2], re[3], imv[3]);
+          gptr<v4f32> d = (gptr<v4f32>)((float2*)P.dst + (long long)j * P.ldd + row0);
+          d[0] = v4f32{re[0], imv[0], re[1], imv[1]};
+          d[1] = v4f32{re[2], imv[2], re[3], imv[3]};
         }
       }
     }
@@ -502,11 +526,11 @@ __device__ __forceinline__ void store_partial(const Work& wk, const Problem& P, 
     const int row = wk.tr * BM + wr * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
     if (has_p) {
       const int pr = trans ? jp : row, pc = trans ? row : jp;
-      if (pr < wk.prows && pc < wk.pcols) wk.partial[(long long)pr * wk.ldp + pc] = ip[e];
+      if (pr < wk.prows && pc < wk.pcols) ((gptr<float>)wk.partial)[(long long)pr * wk.ldp + pc] = ip[e];
     }
     if (has_m) {
       const int pr = trans ? jm : row, pc = trans ? row : jm;
-      if (pr < wk.prows && pc < wk.pcols) wk.partial[(long long)pr * wk.ldp + pc] = im[e];
+      if (pr < wk.prows && pc < wk.pcols) ((gptr<float>)wk.partial)[(long long)pr * wk.ldp + pc] = im[e];
     }
   }
 }

@@ -152,14 +152,6 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
   return dtype == 3 ? ((const double*)p)[idx] : (double)((const float*)p)[idx];
 }
 
-// Global-memory views: pointers that arrive inside descriptors are generic,
-// and generic (flat) accesses also count on the LDS/scalar counter, so every
-// wait for a scalar load would drain them too.
-template <typename T>
-using gptr = __attribute__((address_space(1))) T*;
-typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
-typedef float v4f32 __attribute__((ext_vector_type(4)));
-
 __device__ __forceinline__ PhasorWave load_wave(gptr<const PhasorWave> p) {
   PhasorWave w;
   w.inv_wl = p->inv_wl;
