@@ -317,7 +317,9 @@ def main():
                 'frac': ach / PEAK[args.precision],
                 'traffic': measured_traffic(args.precision, plan.lambda_group, plan.tile_n),
                 'traffic_source': os.path.relpath(TRAFFIC_FILE, ROOT),
-                'kernel': f'gemm_kernel<{args.precision}, intensity> (stage 2)',
+                'kernel': ('gemm_ws_kernel<2> (stage 2: warp-specialised BF16x3, intensity epilogue)'
+                           if args.precision == 'bf16x3' and plan.tile_n == 128
+                           else f'gemm_kernel<{args.precision}, intensity, tile_n={plan.tile_n}> (stage 2)'),
                 'algorithmic_flops_per_launch': flops2, 'avg_launch_ms': stage_ms['stage2'],
                 'timing_mode': stage_ms['mode'], 'stage_sum_ms': stage_sum,
                 'executed_mfma_flops_per_launch': mfma2,
