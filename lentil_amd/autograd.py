@@ -61,6 +61,16 @@ class BroadbandPSFModel:
                             copies=len(self.div))
         self._wide = torch.empty(self.plan.shape_out, dtype=torch.float32, device=self.dev)
         self._gen = 0  # forward calls so far: the plan's operands belong to the last one
+        # the backward reads opd + diversity from the plan's own OPD buffers
+        # (tile 0 of every copy is the whole pupil: the mask is the full grid)
+        for d in range(len(self.div)):
+            ob = self.plan.opd_buffer(0, copy=d)
+            assert tuple(ob.shape) == self.shape and ob.stride(1) == 1
+        # the amplitude last written into the plan and into the backward's
+        # buffer: (source tensor, its version) -- the source is kept, so its
+        # memory cannot be reused by another tensor while it is cached
+        self._amp_in_plan = None
+        self._amp_in_bw = None
         self.alphas = [(self.dx * self.du) / (wl * self.z * self.oversample) for wl in self.wavelengths]
 
     def __call__(self, opd, amp=None):
@@ -69,12 +79,25 @@ class BroadbandPSFModel:
             amp = self.amp
         return _PSFStack.apply(opd, amp, self)
 
+    @staticmethod
+    def _same(cached, t):
+        """Whether ``t`` views the cached source tensor's storage unchanged
+        (same pointer, layout and dtype, no in-place write since)."""
+        if cached is None:
+            return False
+        src, version = cached
+        return (t.data_ptr() == src.data_ptr() and t.dtype == src.dtype and t.shape == src.shape
+                and t.stride() == src.stride() and t._version == version)
+
     # ---------------------------------------------------------------- forward
     def forward_images(self, opd, amp):
-        amp32 = amp.to(torch.float32)  # converted once for every copy
         for d, div in enumerate(self.div):
             torch.add(opd, div, out=self.plan.opd_buffer(0, copy=d))
-            self.plan.set_amplitude(0, amp32, copy=d)
+        if not self._same(self._amp_in_plan, amp):
+            amp32 = amp.to(torch.float32)  # converted once for every copy
+            for d in range(len(self.div)):
+                self.plan.set_amplitude(0, amp32, copy=d)
+            self._amp_in_plan = (amp, amp._version)
         self.plan(self._wide)
         self._gen += 1
         # a fresh tensor: the next forward call overwrites the plan's detector
@@ -91,7 +114,6 @@ class BroadbandPSFModel:
         nd, nw = len(self.div), len(self.wavelengths)
         dev = self.dev
         st = dict(amp=torch.empty((m, n), dtype=torch.float64, device=dev),
-                  tots=torch.empty((nd, m, n), dtype=torch.float64, device=dev),
                   gI=torch.empty((nd, M, M), dtype=torch.float32, device=dev),
                   Y=torch.empty((nd * nw, M, M), dtype=torch.complex64, device=dev),
                   D=torch.empty((nd * nw, m, n), dtype=torch.complex64, device=dev))
@@ -158,20 +180,24 @@ class BroadbandPSFModel:
         M = self.M
         nd, nw = len(self.div), len(self.wavelengths)
         st = self._backward_state(m, n, M)
-        st['amp'].copy_(amp.detach())
-        for d, div in enumerate(self.div):
-            torch.add(opd.detach(), div, out=st['tots'][d])
+        amp = amp.detach()
+        if not self._same(self._amp_in_bw, amp):
+            st['amp'].copy_(amp)
+            self._amp_in_bw = (amp, amp._version)
         st['gI'].copy_(grad_images.detach())
         g_opd = torch.zeros((m, n), dtype=torch.float64, device=self.dev)
         g_amp = torch.zeros((m, n), dtype=torch.float64, device=self.dev) if need_amp else None
         if gen is None or gen != self._gen:
-            self.forward_images(opd.detach(), amp.detach())
+            self.forward_images(opd.detach(), amp)
+        # from here on the plan's operands and OPD buffers (opd + diversity d)
+        # belong to this gradient's forward
         self.plan.fields()
         (adj_plan,) = st['plans']
         _lib.check(lib.mftx_mft_plan_execute(adj_plan), ctx, 'mftx_mft_plan_execute (adjoint)')
         for d in range(nd):
-            _lib.check(lib.mftx_phase_grad_many(ctx, _ptr(st['amp']), _lib.DT_F64, n, _ptr(st['tots'][d]),
-                                                _lib.DT_F64, n, m, n, st['wls'], nw, _ptr(st['D'][d * nw]), n, m * n,
+            tot = self.plan.opd_buffer(0, copy=d)
+            _lib.check(lib.mftx_phase_grad_many(ctx, _ptr(st['amp']), _lib.DT_F64, n, _ptr(tot),
+                                                _lib.DT_F64, tot.stride(0), m, n, st['wls'], nw, _ptr(st['D'][d * nw]), n, m * n,
                                                 _ptr(g_opd), n, _ptr(g_amp) if need_amp else None, n),
                        ctx, 'mftx_phase_grad_many')
         return g_opd, g_amp

@@ -144,3 +144,33 @@ def test_gradient_after_another_forward(L):
     assert rel_l2(h1, g1) < 1e-6
     assert rel_l2(h2, g2) < 1e-6
     model.close()
+
+
+def test_amplitude_written_in_place_is_seen(L):
+    """The model keeps the last amplitude it uploaded and skips the upload
+    while the tensor is unchanged; an in-place write to it is seen by the
+    images and by the gradient (compared with a fresh model given the new
+    values)."""
+    amp, opd, div, wl, wt, geom, target = _case(seed=52)
+    model = _model(L, amp, div, wl, wt, geom, 'bf16x3')
+    T = torch.tensor(target, device='cuda', dtype=torch.float64)
+    a = torch.tensor(amp, device='cuda')
+
+    def run(m, amp_t):
+        x = torch.tensor(opd, device='cuda', requires_grad=True)
+        img = m(x, amp_t)
+        torch.sum((img.double() - T) ** 2).backward()
+        return img.detach().double().cpu().numpy(), x.grad.cpu().numpy()
+
+    i1, g1 = run(model, a)
+    i1b, g1b = run(model, a)                 # cached amplitude: same results
+    assert np.array_equal(i1, i1b) and np.array_equal(g1, g1b)
+    with torch.no_grad():
+        a.mul_(0.5)                           # in place: must be re-uploaded
+    i2, g2 = run(model, a)
+    fresh = _model(L, amp * 0.5, div, wl, wt, geom, 'bf16x3')
+    i3, g3 = run(fresh, torch.tensor(amp * 0.5, device='cuda'))
+    assert rel_l2(i2, i3) < 1e-6 and rel_l2(g2, g3) < 1e-6
+    assert rel_l2(i2, i1) > 0.5               # the images did change (a quarter of the power)
+    model.close()
+    fresh.close()

@@ -371,3 +371,21 @@ def test_tune_cache_file(tmp_path, monkeypatch):
     assert batched.tune_cache_path() == path
     monkeypatch.setenv('LENTIL_AMD_TUNE_CACHE', '')
     assert batched.tune_cache_path() is None
+
+
+def test_retrieval_amplitude_cache_key():
+    """The retrieval model skips re-uploading an amplitude only when the
+    tensor views the cached storage unchanged: an in-place write, another
+    tensor with equal values, or another view of it all count as new."""
+    import torch
+    from lentil_amd.autograd import BroadbandPSFModel
+    same = BroadbandPSFModel._same
+    a = torch.ones((4, 5), dtype=torch.float64)
+    cached = (a, a._version)
+    assert not same(None, a)
+    assert same(cached, a) and same(cached, a.detach())
+    assert not same(cached, a.clone())
+    assert not same(cached, a.t())
+    assert not same(cached, a.float())
+    a.mul_(2.0)
+    assert not same(cached, a) and not same(cached, a.detach())
