@@ -17,6 +17,10 @@ typedef float v2f32 __attribute__((ext_vector_type(2)));
 // wait for a scalar load would drain them too.
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
+// Read-only for the whole launch (descriptor tables): a wave-uniform read
+// through it is a scalar load into SGPRs.
+template <typename T>
+using cptr = __attribute__((address_space(4))) T*;
 
 // --------------------------------------------------------------------------
 // scalar helpers

@@ -152,7 +152,7 @@ __device__ __forceinline__ double load_real(const void* p, int dtype, long long 
   return dtype == 3 ? ((const double*)p)[idx] : (double)((const float*)p)[idx];
 }
 
-__device__ __forceinline__ PhasorWave load_wave(gptr<const PhasorWave> p) {
+__device__ __forceinline__ PhasorWave load_wave(cptr<const PhasorWave> p) {
   PhasorWave w;
   w.inv_wl = p->inv_wl;
   w.fold = p->fold;
@@ -176,7 +176,10 @@ __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, 
   if (u >= phasor_threads(op.rows_pad, op.kpad, op.nwave)) return;
   const int khalves = op.kpad / 8;
   const long long per_group = op.rows_pad * khalves;
-  const int g = (int)(u / per_group);
+  // the wavelength group is wave-uniform: rows_pad is a multiple of 128 and
+  // op.begin of kGroupBlock, so a group spans whole waves; its descriptors
+  // are then scalar loads
+  const int g = __builtin_amdgcn_readfirstlane((int)(u / per_group));
   const long long r = u % per_group;
   const long long row = op.trans ? r % op.rows_pad : r / khalves;
   const int kh = (int)(op.trans ? r / op.rows_pad : r % khalves);
@@ -214,7 +217,7 @@ __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, 
   const int w0 = g * PH_WAVES, nw = min(op.nwave - w0, PH_WAVES);
   PhasorWave pws[PH_WAVES];
 #pragma unroll
-  for (int j = 0; j < PH_WAVES; ++j) pws[j] = load_wave((gptr<const PhasorWave>)op.waves + w0 + min(j, nw - 1));
+  for (int j = 0; j < PH_WAVES; ++j) pws[j] = load_wave((cptr<const PhasorWave>)op.waves + w0 + min(j, nw - 1));
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     a[i] = in[i] ? a[i] : 0.f;
