@@ -195,6 +195,18 @@ def test_dft2_many_vs_oracle(L, precision):
         f = amp * np.exp(2j * np.pi * opd / wl)
         refs.append(0.37 * w * mft.dft2(f, alpha, (M, N), (0.7, -1.3), (0, 0), unitary=False))
         keep.append(F)
+    # float32 amplitude and OPD (the phasor kernel's other load paths; a
+    # phasor op of their own, since the dtypes differ)
+    amp32, opd32 = amp.astype(np.float32), opd.astype(np.float32)
+    ad32, od32 = torch.from_numpy(amp32).cuda(), torch.from_numpy(opd32).cuda()
+    keep += [ad32, od32]
+    for wl in (550e-9, 650e-9):
+        F = torch.empty((M, N), dtype=torch.complex64, device='cuda')
+        entries.append(_mft_entry(L, F, alpha, (0.7, -1.3), (0.0, 0.0), 0.37, amp=ad32, opd=od32, inv_wl=1 / wl,
+                                  w=wd))
+        f = amp32.astype(float) * np.exp(2j * np.pi * opd32.astype(float) / wl)
+        refs.append(0.37 * w * mft.dft2(f, alpha, (M, N), (0.7, -1.3), (0, 0), unitary=False))
+        keep.append(F)
     _many(L, entries, precision)
     outs = [t for t in keep if t.dtype == torch.complex64]
     for i, (F, ref) in enumerate(zip(outs, refs)):
