@@ -721,19 +721,20 @@ typedef __attribute__((address_space(1))) const void gvoid;
 // holds the row's logical unit (l & 15) ^ (row & 15) (row & 15 = 4 j + (l >> 4)).
 struct WsACursor {
   int p, c, nch;              // problem, chunk in it, chunks in it
-  const char* base;           // this lane's row of load 0 at chunk 0
-  long long row4;             // bytes between the rows of consecutive loads
-  int unit[4];                // source byte offset in the row for load j
+  const char* base;           // wave-uniform: the wave's first row (16 w) at chunk 0
+  int voff[4];                // this lane's byte offset from base (at chunk 0) for load j
 };
 
+// The base is wave-uniform (row0 and w are), so it lives in SGPRs and each
+// load adds only the chunk offset to it there; a lane's offset within the
+// wave's 16 rows fits 32 bits (15 rows of a row stride below 128 MiB).
 __device__ __forceinline__ void ws_a_set(WsACursor& ac, const Problem& P, long long row0, int lane, int w) {
   const long long ldb = P.lda * 2;  // bytes per A row
   const int rr = lane >> 4, pu = lane & 15;
   ac.nch = P.kpad / ws::BKC;
-  ac.row4 = 4 * ldb;
-  ac.base = (const char*)P.a + (row0 + 16 * w + rr) * ldb;
+  ac.base = (const char*)P.a + (row0 + 16 * w) * ldb;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) ac.unit[j] = 16 * (pu ^ (4 * j + rr));
+  for (int j = 0; j < 4; ++j) ac.voff[j] = (int)((4 * j + rr) * ldb) + 16 * (pu ^ (4 * j + rr));
 }
 
 // Direct-to-LDS load j (rows 16 w + 4 j .. + 3) of the cursor's chunk into A
@@ -749,10 +750,10 @@ __device__ __forceinline__ void ws_a_set(WsACursor& ac, const Problem& P, long l
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void ws_a_load1(ws::Lds& L, int ab, int w, const WsACursor& ac, int j) {
-  const char* src = ac.base + (long long)ac.c * 256 + j * ac.row4 + ac.unit[j];
+  const char* src = ac.base + (long long)ac.c * 256;  // wave-uniform
   const unsigned dst = (unsigned)(size_t)(ws::lds_void*)&L.a[ab][16 * w + 4 * j][0];
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
-               :: "v"(src), "s"(__builtin_amdgcn_readfirstlane(dst)) : "memory", "m0");
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
+               :: "v"(ac.voff[j]), "s"(src), "s"(dst) : "memory", "m0");
 }
 #pragma clang diagnostic pop
 
@@ -805,7 +806,7 @@ __device__ __forceinline__ void gemm_ws_body(const Work& wk, const Problem* __re
                                              const double* __restrict__ chunk_x, ws::Lds& L) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wid = tid >> 6;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform, in an SGPR
   int nchunks = 0;  // every problem's K loop back to back (uniform)
   for (int p = wk.pbeg; p < wk.pend; ++p) nchunks += probs[p].kpad / ws::BKC;
 
