@@ -307,7 +307,7 @@ def main():
     # of replayed prefix graphs, after the clocks have settled again
     scratch = torch.empty_like(out)
     warm_up(lambda: plan(scratch), 0, torch.cuda.synchronize)
-    stage_ms = plan.profile(scratch, repeats=20)
+    stage_ms = plan.profile(scratch, repeats=60)
     stage_sum = sum(stage_ms[k] for k in ('phasor', 'stage1', 'stage2', 'reduce'))
     flops2 = plan.info['gemm_flops'][1]
     ach = flops2 / (stage_ms['stage2'] * 1e-3)
@@ -329,10 +329,17 @@ def main():
                            / PEAK[args.precision]}}
     print(f'[bench] stages ms {stage_ms}', file=sys.stderr, flush=True)
     # HBM-bound field kernels: algorithmic bytes / event-timed duration
-    field_kernels = {k: {'ms': stage_ms[k], 'bytes': plan.info[f'bytes_{k}'],
-                         'gb_per_s': plan.info[f'bytes_{k}'] / (stage_ms[k] * 1e-3) / 1e9,
-                         'frac_of_hbm_peak': plan.info[f'bytes_{k}'] / (stage_ms[k] * 1e-3) / HBM_PEAK}
-                     for k in ('phasor', 'reduce')}
+    # a few-microsecond kernel is within the noise of the prefix-graph
+    # differences; its rate is reported only above 1 us (the kernel traces in
+    # profiles/ time it directly)
+    def field(k):
+        ms, nbytes = stage_ms[k], plan.info[f'bytes_{k}']
+        if ms < 1e-3:
+            return {'ms': ms, 'bytes': nbytes, 'gb_per_s': None, 'frac_of_hbm_peak': None,
+                    'note': 'below the resolution of the prefix-graph differences'}
+        return {'ms': ms, 'bytes': nbytes, 'gb_per_s': nbytes / (ms * 1e-3) / 1e9,
+                'frac_of_hbm_peak': nbytes / (ms * 1e-3) / HBM_PEAK}
+    field_kernels = {k: field(k) for k in ('phasor', 'reduce')}
 
     extra = {}
     if tuned:
