@@ -299,7 +299,8 @@ __device__ __forceinline__ void load_weights(const Problem& P, const long long (
 // times the next stage's fold).
 template <int PREC, int EPI>
 __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& wk, int BS, int wr, int wc, int lane,
-                                                 const f32x16 (&acc)[4], f32x16 (&ip), f32x16 (&im)) {
+                                                 const f32x16 (&acc)[4], f32x16 (&ip), f32x16 (&im),
+                                                 const double* __restrict__ chunk_x) {
   const int s = wk.tc * BS + wc * 32 + (lane & 31);
   const int N = P.ncols;
   const int jp = P.cp + s, jm = P.cm - s;
@@ -383,6 +384,13 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
     // (v_permlane32_swap: the high half of the first operand trades places
     // with the low half of the second) gives lane l K 0-7 and lane l+32 K
     // 8-15, so each lane stores 16 bytes of every plane.
+    // The fold of row rb + 8 g + e (rb this lane's first row of the block):
+    // one FP64-phase evaluation for rb, times cis(fold (8 g + e)) from the
+    // problem's 16-entry table, entry 4 g + e (every packed-store problem
+    // carries one; wave-uniform reads, scalar loads).
+    const float2* __restrict__ ftab = (const float2*)(chunk_x + P.fold_tab);
+    const int rb = wk.tr * BM + wr * 32 + 4 * (lane >> 5);
+    const float2 fb = cis_turns(P.fold * (P.fold_x0 + row_f64(rb)));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int kb = wk.tr * BM + wr * 32 + 16 * h;  // first K of the block (wave-uniform)
@@ -390,13 +398,12 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
       uint32_t w[2][2][4][2];  // [group of the pair][side][plane][word]
 #pragma unroll
       for (int gg = 0; gg < 2; ++gg) {
-        const int g = 2 * h + gg;
-        const int row0 = wk.tr * BM + wr * 32 + 8 * g + 4 * (lane >> 5);
+        const int g = 2 * h + gg;  // this lane's rows rb + 8 g + e
         float v[2][2][4];  // [side][re, im][e]
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float pr = acc[0][4 * g + e], pi = acc[1][4 * g + e], qr = acc[2][4 * g + e], qi = acc[3][4 * g + e];
-          const float2 f = cis_turns(P.fold * (P.fold_x0 + row_f64(row0 + e)));
+          const float2 f = cmul(fb, ftab[4 * g + e]);  // cis(fold (8 g + e)), stored at 4 g + e
           const float ar = (pr - qi) * sc, ai = (pi + qr) * sc, br = (pr + qi) * sc, bi = (pi - qr) * sc;
           v[0][0][e] = ar * f.x - ai * f.y;
           v[0][1][e] = ar * f.y + ai * f.x;
@@ -636,7 +643,7 @@ __device__ __forceinline__ void gemm_body(const Work& wk, const Problem* __restr
     __syncthreads();
     cur ^= 1;
     if (++cs == cn) {
-      problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im);
+      problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im, chunk_x);
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
       if (++pc < wk.pend) {
@@ -969,7 +976,7 @@ __device__ __forceinline__ void gemm_ws_body(const Work& wk, const Problem* __re
     an = an == 2 ? 0 : an + 1;
     ab = ab == 2 ? 0 : ab + 1;
     if (++cs == cn) {
-      problem_epilogue<PREC_BF16X3, EPI>(probs[pc], wk, ws::BS, wr, wc, lane, acc, ip, im);
+      problem_epilogue<PREC_BF16X3, EPI>(probs[pc], wk, ws::BS, wr, wc, lane, acc, ip, im, chunk_x);
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
       if (++pc < wk.pend) {

@@ -70,7 +70,8 @@ struct Problem {
   const void* wmap;     // EPI_STORE_CPLX: optional real map multiplied into each stored value,
   long long ldw;        //   indexed like the destination (row stride ldw)
   int wdtype;           //   1 float32, 3 float64
-  int pad2_;
+  int fold_tab;         // EPI_STORE_PACKED_T: index in chunk_x of 16 float2 factors cis(fold * (8 g + e)),
+                        //   g, e = 0..3 (one per double slot); read by the BF16x3 packed store
 };
 
 // Per-wavelength parameters of a phasor launch: the phase opd * inv_wl and the
