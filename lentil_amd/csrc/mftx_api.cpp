@@ -114,17 +114,18 @@ void push_chunks(std::vector<double>& cx, int kpad, double x0) {
 }
 
 // The factors by which the next stage's fold steps from a row of a 32-row
-// block to rows 8 g + e further on (g, e = 0..3): cis(fold * (8 g + e)),
-// appended to the chunk-origin table as float2, one per double slot, and
-// evaluated here in double precision.  Returns their index, which is never 0
-// (a problem's chunk origins come first).
-int push_fold_table(std::vector<double>& cx, double fold) {
+// block to rows 8 g + e further on (g, e = 0..3), times the store's scale:
+// scale * cis(fold * (8 g + e)), appended to the chunk-origin table as
+// float2, one per double slot, and evaluated here in double precision.
+// Returns their index, which is never 0 (a problem's chunk origins come
+// first).
+int push_fold_table(std::vector<double>& cx, double fold, double scale) {
   const int at = (int)cx.size();
   for (int g = 0; g < 4; ++g)
     for (int e = 0; e < 4; ++e) {
       const double t = fold * (double)(8 * g + e);
       const double turns = t - std::nearbyint(t);
-      const float v[2] = {(float)std::cos(2.0 * M_PI * turns), (float)std::sin(2.0 * M_PI * turns)};
+      const float v[2] = {(float)(scale * std::cos(2.0 * M_PI * turns)), (float)(scale * std::sin(2.0 * M_PI * turns))};
       double slot;
       std::memcpy(&slot, v, sizeof slot);
       cx.push_back(slot);
@@ -332,7 +333,7 @@ static int run_single(mftx_ctx* ctx, int prec, int epi, int bn, Problem p, doubl
   std::vector<double> cx;
   push_chunks(cx, p.kpad, x0);
   p.chunk_begin = 0;
-  if (epi == EPI_STORE_PACKED_T) p.fold_tab = push_fold_table(cx, p.fold);
+  if (epi == EPI_STORE_PACKED_T) p.fold_tab = push_fold_table(cx, p.fold, p.scale);
   const size_t off_cx = round_up(sizeof(Problem), 16);
   const size_t off_w = round_up(off_cx + sizeof(double) * cx.size(), 16);
   std::vector<char> host(off_w + sizeof(Work) * works.size());
@@ -549,7 +550,7 @@ static void many_build(const mftx_mft* mfts, int32_t count, int prec, ManyBatch&
     a.ldd = packed_lda(prec, g.kpad2);
     a.fold = g.g2.fold;
     a.fold_x0 = g.x02;
-    a.fold_tab = push_fold_table(cx, a.fold);
+    a.fold_tab = push_fold_table(cx, a.fold, a.scale);
 
     Problem& b = p2[i];
     b.lda = packed_lda(prec, g.kpad2);
@@ -1462,7 +1463,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       // the stage-2 fold for this member's K segment; the stage-1 fold goes to the phasor
       q.fold = g2.fold;
       q.fold_x0 = x02;
-      q.fold_tab = push_fold_table(cx, q.fold);
+      q.fold_tab = push_fold_table(cx, q.fold, q.scale);
       stage1_fold[{P.tile, cf ? 1 : 0, P.wave}] = {g1.fold, x01};
       const size_t elem = prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(uint16_t);
       patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)op.slot.at(P.wave) * packed_bytes(prec, op.rows, op.kpad),

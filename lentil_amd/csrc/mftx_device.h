@@ -73,8 +73,11 @@ __device__ __forceinline__ float bf16_lo_to_f32(uint32_t u) { return __uint_as_f
 __device__ __forceinline__ float bf16_hi_to_f32(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
 // Split two floats into (hi pair, lo pair) with x = hi + lo + O(2^-17 |x|).
+// (The empty asm keeps the compiler from looking through the pack, where it
+// would convert a to bf16 a second time instead of shifting the pair.)
 __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
   hi = pack_bf16(a, b);
+  asm("" : "+v"(hi));
   lo = pack_bf16(a - bf16_lo_to_f32(hi), b - bf16_hi_to_f32(hi));
 }
 

@@ -384,10 +384,11 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
     // (v_permlane32_swap: the high half of the first operand trades places
     // with the low half of the second) gives lane l K 0-7 and lane l+32 K
     // 8-15, so each lane stores 16 bytes of every plane.
-    // The fold of row rb + 8 g + e (rb this lane's first row of the block):
-    // one FP64-phase evaluation for rb, times cis(fold (8 g + e)) from the
-    // problem's 16-entry table, entry 4 g + e (every packed-store problem
-    // carries one; wave-uniform reads, scalar loads).
+    // The fold of row rb + 8 g + e (rb this lane's first row of the block),
+    // with the store's scale: one FP64-phase evaluation for rb, times
+    // scale * cis(fold (8 g + e)) from the problem's 16-entry table, entry
+    // 4 g + e (every packed-store problem carries one; wave-uniform reads,
+    // scalar loads).
     const float2* __restrict__ ftab = (const float2*)(chunk_x + P.fold_tab);
     const int rb = wk.tr * BM + wr * 32 + 4 * (lane >> 5);
     const float2 fb = cis_turns(P.fold * (P.fold_x0 + row_f64(rb)));
@@ -403,8 +404,9 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float pr = acc[0][4 * g + e], pi = acc[1][4 * g + e], qr = acc[2][4 * g + e], qi = acc[3][4 * g + e];
-          const float2 f = cmul(fb, ftab[4 * g + e]);  // cis(fold (8 g + e)), stored at 4 g + e
-          const float ar = (pr - qi) * sc, ai = (pi + qr) * sc, br = (pr + qi) * sc, bi = (pi - qr) * sc;
+          // scale * cis(fold (8 g + e)), stored at 4 g + e
+          const float2 f = cmul(fb, ftab[4 * g + e]);
+          const float ar = pr - qi, ai = pi + qr, br = pr + qi, bi = pi - qr;
           v[0][0][e] = ar * f.x - ai * f.y;
           v[0][1][e] = ar * f.y + ai * f.x;
           v[1][0][e] = br * f.x - bi * f.y;
