@@ -113,26 +113,6 @@ void push_chunks(std::vector<double>& cx, int kpad, double x0) {
   for (int c = 0; c < kpad / BK; ++c) cx.push_back(x0 + (double)(c * BK));
 }
 
-// The factors by which the next stage's fold steps from a row of a 32-row
-// block to rows 8 g + e further on (g, e = 0..3), times the store's scale:
-// scale * cis(fold * (8 g + e)), appended to the chunk-origin table as
-// float2, one per double slot, and evaluated here in double precision.
-// Returns their index, which is never 0 (a problem's chunk origins come
-// first).
-int push_fold_table(std::vector<double>& cx, double fold, double scale) {
-  const int at = (int)cx.size();
-  for (int g = 0; g < 4; ++g)
-    for (int e = 0; e < 4; ++e) {
-      const double t = fold * (double)(8 * g + e);
-      const double turns = t - std::nearbyint(t);
-      const float v[2] = {(float)(scale * std::cos(2.0 * M_PI * turns)), (float)(scale * std::sin(2.0 * M_PI * turns))};
-      double slot;
-      std::memcpy(&slot, v, sizeof slot);
-      cx.push_back(slot);
-    }
-  return at;
-}
-
 }  // namespace
 
 struct mftx_ctx {
@@ -333,7 +313,6 @@ static int run_single(mftx_ctx* ctx, int prec, int epi, int bn, Problem p, doubl
   std::vector<double> cx;
   push_chunks(cx, p.kpad, x0);
   p.chunk_begin = 0;
-  if (epi == EPI_STORE_PACKED_T) p.fold_tab = push_fold_table(cx, p.fold, p.scale);
   const size_t off_cx = round_up(sizeof(Problem), 16);
   const size_t off_w = round_up(off_cx + sizeof(double) * cx.size(), 16);
   std::vector<char> host(off_w + sizeof(Work) * works.size());
@@ -473,6 +452,8 @@ static void many_build(const mftx_mft* mfts, int32_t count, int prec, ManyBatch&
       op.lds = q.ldf;
       op.fold = g.g1.fold;
       op.x0 = g.x01;
+      op.fold2 = g.g2.fold;
+      op.x02 = g.x02;
       op.dst = nullptr;  // rebased below
       op.rows_pad = g.rows1;
       op.kpad = g.kpad1;
@@ -518,7 +499,7 @@ static void many_build(const mftx_mft* mfts, int32_t count, int prec, ManyBatch&
       phops.push_back(op);
     }
     phops.back().nwave += 1;
-    waves.push_back(PhasorWave{q.inv_wl, g.g1.fold, g.x01});
+    waves.push_back(PhasorWave{q.inv_wl, g.g1.fold, g.x01, g.g2.fold, g.x02});
     wave_of.push_back((int)waves.size() - 1);
   }
   if (!phops.empty())
@@ -548,9 +529,6 @@ static void many_build(const mftx_mft* mfts, int32_t count, int prec, ManyBatch&
     a.scale = 1.0f;
     a.limit = g.kpad2;
     a.ldd = packed_lda(prec, g.kpad2);
-    a.fold = g.g2.fold;
-    a.fold_x0 = g.x02;
-    a.fold_tab = push_fold_table(cx, a.fold, a.scale);
 
     Problem& b = p2[i];
     b.lda = packed_lda(prec, g.kpad2);
@@ -836,8 +814,10 @@ int mftx::dft2_on_workspace(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m
   const double xr0 = -centre(m) + offset_r, xc0 = -centre(n) + offset_c;
   const double fold1 = g1.fold, fold2 = g2.fold;
   const double x01 = col_first ? xr0 : xc0, x02 = col_first ? xc0 : xr0;
-  HIPCHK(ctx, launch_pack(prec, f, f_dtype, m, n, ldf, col_first ? 1 : 0, fold1, x01, a1, rows1, kpad1,
-                          ctx->stream));
+  // the operand carries both folds: stage 1's along K, stage 2's along its
+  // rows (stage 1's output rows, stage 2's K)
+  HIPCHK(ctx, launch_pack(prec, f, f_dtype, m, n, ldf, col_first ? 1 : 0, fold1, x01, fold2, x02, a1, rows1,
+                          kpad1, ctx->stream));
 
   Problem p1 = {};
   p1.a = a1;
@@ -854,8 +834,6 @@ int mftx::dft2_on_workspace(mftx_ctx* ctx, const void* f, int f_dtype, int64_t m
   p1.limit = kpad2;
   p1.dst = a2;
   p1.ldd = packed_lda(prec, kpad2);
-  p1.fold = fold2;
-  p1.fold_x0 = x02;
   rc = run_single(ctx, prec, EPI_STORE_PACKED_T, bn, p1, x01, rows1);
   if (rc) return rc;
 
@@ -1424,7 +1402,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
 
   // ---- descriptors ---------------------------------------------------------------
   std::vector<Problem> p1, p2;
-  std::map<std::tuple<int, int, int>, std::pair<double, double>> stage1_fold;  // (tile, orientation, wave) -> fold, x0
+  // (tile, orientation, wave) -> the operand's folds: stage 1 (fold, x0) along K, stage 2 (fold, x0) along rows
+  std::map<std::tuple<int, int, int>, std::array<double, 4>> stage1_fold;
   std::vector<double> cx;
   std::vector<std::vector<Work>> clus1, clus2;
   struct Patch { int idx; size_t a, dst; };  // workspace offsets to rebase after allocation
@@ -1461,10 +1440,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       q.limit = kp2;
       q.ldd = packed_lda(prec, c.ktot);
       // the stage-2 fold for this member's K segment; the stage-1 fold goes to the phasor
-      q.fold = g2.fold;
-      q.fold_x0 = x02;
-      q.fold_tab = push_fold_table(cx, q.fold, q.scale);
-      stage1_fold[{P.tile, cf ? 1 : 0, P.wave}] = {g1.fold, x01};
+      stage1_fold[{P.tile, cf ? 1 : 0, P.wave}] = {g1.fold, x01, g2.fold, x02};
       const size_t elem = prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(uint16_t);
       patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)op.slot.at(P.wave) * packed_bytes(prec, op.rows, op.kpad),
                              c.abuf + (size_t)c.kbeg[j] * elem});
@@ -1654,7 +1630,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   std::vector<PhasorWave> iw(wave_slots.size());
   for (size_t i = 0; i < wave_slots.size(); ++i) {
     const auto& fx = stage1_fold.at(wave_slots[i]);
-    iw[i] = PhasorWave{1.0 / wavelengths[std::get<2>(wave_slots[i])], fx.first, fx.second};
+    iw[i] = PhasorWave{1.0 / wavelengths[std::get<2>(wave_slots[i])], fx[0], fx[1], fx[2], fx[3]};
   }
 
   plan->works1 = (Work*)(base + o_w1);

@@ -266,15 +266,6 @@ __device__ __forceinline__ void sblock_bf16(const bf16x8_t (&a)[4], const bf16x8
   }
 }
 
-// A C row index as FP64, converted where it is used.  The row indices of the
-// epilogue do not change along the K loop, so the compiler would hoist their
-// conversions out of it and spill them; reloading a spill at a problem end
-// waits for every load in flight (the operand loads of the next chunks too).
-__device__ __forceinline__ double row_f64(int r) {
-  asm volatile("" : "+v"(r));
-  return (double)r;
-}
-
 // Weight-map values at map offsets idx[0..n-1] (1 without a map).  No load
 // sits behind a branch, so they are in flight together.
 template <int NW>
@@ -295,12 +286,12 @@ __device__ __forceinline__ void load_weights(const Problem& P, const long long (
 // 8 (e / 4) + 4 (lane / 32) + e % 4 of its 32-row block and slot lane % 32;
 // the slot's two output columns are jp = cp + s (P + iQ) and jm = cm - s
 // (P - iQ).  Intensity accumulates w |C|^2 for both in registers; the store
-// forms write C * scale (complex, or packed transposed as the next stage's A',
-// times the next stage's fold).
+// forms write C * scale (complex, or packed transposed as the next stage's A';
+// the next stage's fold is already in C, from the phase of this stage's
+// operand rows).
 template <int PREC, int EPI>
 __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& wk, int BS, int wr, int wc, int lane,
-                                                 const f32x16 (&acc)[4], f32x16 (&ip), f32x16 (&im),
-                                                 const double* __restrict__ chunk_x) {
+                                                 const f32x16 (&acc)[4], f32x16 (&ip), f32x16 (&im)) {
   const int s = wk.tc * BS + wc * 32 + (lane & 31);
   const int N = P.ncols;
   const int jp = P.cp + s, jm = P.cm - s;
@@ -384,14 +375,6 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
     // (v_permlane32_swap: the high half of the first operand trades places
     // with the low half of the second) gives lane l K 0-7 and lane l+32 K
     // 8-15, so each lane stores 16 bytes of every plane.
-    // The fold of row rb + 8 g + e (rb this lane's first row of the block),
-    // with the store's scale: one FP64-phase evaluation for rb, times
-    // scale * cis(fold (8 g + e)) from the problem's 16-entry table, entry
-    // 4 g + e (every packed-store problem carries one; wave-uniform reads,
-    // scalar loads).
-    const float2* __restrict__ ftab = (const float2*)(chunk_x + P.fold_tab);
-    const int rb = wk.tr * BM + wr * 32 + 4 * (lane >> 5);
-    const float2 fb = cis_turns(P.fold * (P.fold_x0 + row_f64(rb)));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int kb = wk.tr * BM + wr * 32 + 16 * h;  // first K of the block (wave-uniform)
@@ -399,18 +382,15 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
       uint32_t w[2][2][4][2];  // [group of the pair][side][plane][word]
 #pragma unroll
       for (int gg = 0; gg < 2; ++gg) {
-        const int g = 2 * h + gg;  // this lane's rows rb + 8 g + e
+        const int g = 2 * h + gg;
         float v[2][2][4];  // [side][re, im][e]
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float pr = acc[0][4 * g + e], pi = acc[1][4 * g + e], qr = acc[2][4 * g + e], qi = acc[3][4 * g + e];
-          // scale * cis(fold (8 g + e)), stored at 4 g + e
-          const float2 f = cmul(fb, ftab[4 * g + e]);
-          const float ar = pr - qi, ai = pi + qr, br = pr + qi, bi = pi - qr;
-          v[0][0][e] = ar * f.x - ai * f.y;
-          v[0][1][e] = ar * f.y + ai * f.x;
-          v[1][0][e] = br * f.x - bi * f.y;
-          v[1][1][e] = br * f.y + bi * f.x;
+          v[0][0][e] = (pr - qi) * sc;
+          v[0][1][e] = (pi + qr) * sc;
+          v[1][0][e] = (pr + qi) * sc;
+          v[1][1][e] = (pi - qr) * sc;
         }
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
@@ -484,17 +464,6 @@ __device__ __forceinline__ void problem_epilogue(const Problem& P, const Work& w
         }
       }
     } else if (row0 < P.limit) {  // EPI_STORE_PACKED_T: destination row = output column, K = C row
-      // the next stage's fold along its K index (this C's row)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float2 f = cis_turns(P.fold * (P.fold_x0 + row_f64(row0 + e)));
-        float t = pre[e] * f.x - pim[e] * f.y;
-        pim[e] = pre[e] * f.y + pim[e] * f.x;
-        pre[e] = t;
-        t = mre[e] * f.x - mim[e] * f.y;
-        mim[e] = mre[e] * f.y + mim[e] * f.x;
-        mre[e] = t;
-      }
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
         const int j = side ? jm : jp;
@@ -645,7 +614,7 @@ __device__ __forceinline__ void gemm_body(const Work& wk, const Problem* __restr
     __syncthreads();
     cur ^= 1;
     if (++cs == cn) {
-      problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im, chunk_x);
+      problem_epilogue<PREC, EPI>(probs[pc], wk, BS, wr, wc, lane, acc, ip, im);
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
       if (++pc < wk.pend) {
@@ -978,7 +947,7 @@ __device__ __forceinline__ void gemm_ws_body(const Work& wk, const Problem* __re
     an = an == 2 ? 0 : an + 1;
     ab = ab == 2 ? 0 : ab + 1;
     if (++cs == cn) {
-      problem_epilogue<PREC_BF16X3, EPI>(probs[pc], wk, ws::BS, wr, wc, lane, acc, ip, im, chunk_x);
+      problem_epilogue<PREC_BF16X3, EPI>(probs[pc], wk, ws::BS, wr, wc, lane, acc, ip, im);
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] = f32x16{};
       if (++pc < wk.pend) {

@@ -65,21 +65,23 @@ struct Problem {
   int trans;            // EPI_STORE_CPLX / EPI_INTENSITY: write C transposed
   void* dst;            // store destination (EPI_STORE_*)
   long long ldd;        // destination row stride (elements of its type)
-  double fold;          // EPI_STORE_PACKED_T: C row r is stored times cis(fold * (fold_x0 + r)),
-  double fold_x0;       //   the next stage's fold
   const void* wmap;     // EPI_STORE_CPLX: optional real map multiplied into each stored value,
   long long ldw;        //   indexed like the destination (row stride ldw)
   int wdtype;           //   1 float32, 3 float64
-  int fold_tab;         // EPI_STORE_PACKED_T: index in chunk_x of 16 float2 factors cis(fold * (8 g + e)),
-                        //   g, e = 0..3 (one per double slot); read by the BF16x3 packed store
+  int pad2_;
 };
 
-// Per-wavelength parameters of a phasor launch: the phase opd * inv_wl and the
-// stage-1 fold cis(fold * (x0 + k)) along the operand's K index k.
+// Per-wavelength parameters of a phasor launch: the phase opd * inv_wl, the
+// stage-1 fold cis(fold * (x0 + k)) along the operand's K index k, and the
+// stage-2 fold cis(fold2 * (x02 + r)) along its row index r (stage 1's C row r
+// is the operand's row r times the row's phase, so the stage-1 output comes
+// out folded for stage 2).
 struct PhasorWave {
   double inv_wl;
   double fold;
   double x0;
+  double fold2;
+  double x02;
 };
 
 // A workgroup's unit of work: problems [pbeg, pend) on tile (tr, tc) of C.
@@ -135,7 +137,8 @@ struct PackOp {
   const void* src;
   int dtype, trans;
   long long m, n, lds;
-  double fold, x0;
+  double fold, x0;      // stage-1 fold along K
+  double fold2, x02;    // stage-2 fold along the packed rows (see PhasorWave)
   void* dst;
   long long rows_pad;
   int kpad, pad_;
@@ -146,10 +149,11 @@ struct PackOp {
 // bn: output columns per workgroup (64 or 128, see valid_bn).
 hipError_t launch_gemm(int prec, int epi, int bn, const Work* work, int nwork, const Problem* probs,
                        const double* chunk_x, hipStream_t s);
-// packed A' = src (or src^T) times the fold cis(fold * (x0 + k)) along K
+// packed A' = src (or src^T) times the folds cis(fold * (x0 + k)) along K and
+// cis(fold2 * (x02 + row)) along the packed rows
 hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n,
-                       long long lds, int trans, double fold, double x0, void* dst, long long rows_pad, int kpad,
-                       hipStream_t s);
+                       long long lds, int trans, double fold, double x0, double fold2, double x02, void* dst,
+                       long long rows_pad, int kpad, hipStream_t s);
 // every op of a plan in one launch (ops_dev on the device, sorted by begin)
 hipError_t launch_phasor_group(int prec, const PhasorOp* ops_dev, int nops, long long total_threads, hipStream_t s);
 // every complex pack op in one launch (ops_dev on the device, sorted by begin)

@@ -49,15 +49,18 @@ __device__ __forceinline__ void store_packed16(int prec, void* dst, long long ld
 // One thread per (row, 16-wide K block) of the packed destination.  The
 // source is m x n with row stride lds; packed element (row, k) is source
 // (row, k), or source (k, row) when trans (the packed operand is then n x m),
-// times the fold cis(fold * (x0 + k)).  Everything outside the source is zero.
+// times the folds cis(fold * (x0 + k) + fold2 * (x02 + row)).  Everything
+// outside the source is zero.
 __global__ void pack_kernel(int prec, const void* __restrict__ src, int dtype, long long m, long long n,
-                            long long lds, int trans, double fold, double x0, void* __restrict__ dst,
-                            long long rows_pad, int kpad) {
+                            long long lds, int trans, double fold, double x0, double fold2, double x02,
+                            void* __restrict__ dst, long long rows_pad, int kpad) {
   const int kblocks = kpad / 16;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= rows_pad * kblocks) return;
   const long long row = t / kblocks;
   const int kb = (int)(t % kblocks);
+  const bool folded = fold != 0.0 || fold2 != 0.0;
+  const double row_phase = fold2 * (x02 + (double)row);
   float2 v[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
@@ -65,7 +68,7 @@ __global__ void pack_kernel(int prec, const void* __restrict__ src, int dtype, l
     const long long r = trans ? k : row, c = trans ? row : k;
     if (r < m && c < n) {
       const float2 e = load_elem(src, dtype, r * lds + c);
-      v[i] = fold == 0.0 ? e : cmul(e, cis_turns(fold * (x0 + (double)k)));
+      v[i] = folded ? cmul(e, cis_turns(__builtin_fma(fold, x0 + (double)k, row_phase))) : e;
     } else {
       v[i] = make_float2(0.f, 0.f);
     }
@@ -74,12 +77,13 @@ __global__ void pack_kernel(int prec, const void* __restrict__ src, int dtype, l
 }
 
 hipError_t launch_pack(int prec, const void* src, int src_dtype, long long m, long long n, long long lds,
-                       int trans, double fold, double x0, void* dst, long long rows_pad, int kpad, hipStream_t s) {
+                       int trans, double fold, double x0, double fold2, double x02, void* dst, long long rows_pad,
+                       int kpad, hipStream_t s) {
   const long long total = rows_pad * (kpad / 16);
   if (total <= 0) return hipSuccess;
   const int bs = 256;
   hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, s, prec, src,
-                     src_dtype, m, n, lds, trans, fold, x0, dst, rows_pad, kpad);
+                     src_dtype, m, n, lds, trans, fold, x0, fold2, x02, dst, rows_pad, kpad);
   return hipGetLastError();
 }
 
@@ -109,6 +113,8 @@ __global__ void pack_group_kernel(int prec, const PackOp* __restrict__ ops, int 
   if (u >= op.rows_pad * npairs) return;
   const long long row = op.trans ? u % op.rows_pad : u / npairs;
   const int kp = (int)(op.trans ? u / op.rows_pad : u % npairs);
+  const bool folded = op.fold != 0.0 || op.fold2 != 0.0;
+  const double row_phase = op.fold2 * (op.x02 + (double)row);
   float2 v[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -116,7 +122,7 @@ __global__ void pack_group_kernel(int prec, const PackOp* __restrict__ ops, int 
     const long long r = op.trans ? k : row, c = op.trans ? row : k;
     if (r < op.m && c < op.n) {
       const float2 e = load_elem(op.src, op.dtype, r * op.lds + c);
-      v[i] = op.fold == 0.0 ? e : cmul(e, cis_turns(op.fold * (op.x0 + (double)k)));
+      v[i] = folded ? cmul(e, cis_turns(__builtin_fma(op.fold, op.x0 + (double)k, row_phase))) : e;
     } else {
       v[i] = make_float2(0.f, 0.f);
     }
@@ -157,6 +163,8 @@ __device__ __forceinline__ PhasorWave load_wave(cptr<const PhasorWave> p) {
   w.inv_wl = p->inv_wl;
   w.fold = p->fold;
   w.x0 = p->x0;
+  w.fold2 = p->fold2;
+  w.x02 = p->x02;
   return w;
 }
 
@@ -164,8 +172,9 @@ __device__ __forceinline__ PhasorWave load_wave(cptr<const PhasorWave> p) {
 // wavelengths), every phasor operand of a plan in one launch (the op is found
 // by a block-uniform search).  The 8 amp/opd values are read once and the
 // packed operand of the m x n tile (n x m when trans) is written for each
-// wavelength of the group, with that wavelength's stage-1 fold along K in the
-// same phase:  amp * cis(opd * inv_wl + fold * (x0 + k)).  Consecutive threads
+// wavelength of the group, with that wavelength's stage-1 fold along K and
+// stage-2 fold along the rows r in the same phase:
+// amp * cis(opd * inv_wl + fold * (x0 + k) + fold2 * (x02 + r)).  Consecutive threads
 // take consecutive source columns (half blocks when not trans, rows of the
 // transposed operand when trans), so the source reads of a wave are
 // contiguous; the two halves of a block store one 32-byte plane together.
@@ -231,7 +240,9 @@ __global__ void phasor_group_kernel(int prec, const PhasorOp* __restrict__ ops, 
     for (int j = 0; j < PH_WAVES; ++j) {
       if (j >= nw) break;
       const PhasorWave pw = pws[j];
-      const double f0 = pw.fold * (pw.x0 + (double)kb * 16);
+      // both folds' phase at K = 16 kb of this row: the stage-1 fold along
+      // K, the stage-2 fold along the rows
+      const double f0 = __builtin_fma(pw.fold, pw.x0 + (double)kb * 16, pw.fold2 * (pw.x02 + (double)row));
       float2 v[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
