@@ -749,13 +749,25 @@ struct WsB {
   float2 q[8];
 };
 
+// Per problem: the slot's phase rate and q[i] = cis(4 alpha i y), i = 0..7,
+// from one FP64-phase evaluation and a product tree of depth 3 (about 3 ulp):
+// eight full evaluations at every problem start made the twiddle waves late
+// for that chunk's barrier.
 __device__ __forceinline__ void ws_b_set(WsB& b, const Problem& P, int slot) {
   b.alpha4 = 4.0 * P.alpha;
   b.y = P.y0 + (double)slot;
   b.nch = P.kpad / ws::BKC;
   b.cb = P.chunk_begin;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) b.q[i] = cis_quarter(b.alpha4 * ((double)i * b.y));
+  const float2 q1 = cis_quarter(b.alpha4 * b.y);
+  const float2 q2 = cmul(q1, q1), q4 = cmul(q2, q2);
+  b.q[0] = make_float2(1.0f, 0.0f);
+  b.q[1] = q1;
+  b.q[2] = q2;
+  b.q[3] = cmul(q2, q1);
+  b.q[4] = q4;
+  b.q[5] = cmul(q4, q1);
+  b.q[6] = cmul(q4, q2);
+  b.q[7] = cmul(q4, b.q[3]);
 }
 
 // Twiddles of K group g (K 8g .. 8g+7 of the chunk) of this thread's slot into B buffer bb.
