@@ -878,11 +878,10 @@ __device__ __forceinline__ void gemm_ws_body(const Work& wk, const Problem* __re
     ws_a_load1(L, an, wid, ac, j);
     __builtin_amdgcn_sched_barrier(0);
   };
-  // The intensity epilogue (stage 2) reads chunk s + 1's first fragments
-  // under chunk s's last product group, across the chunk's barrier.  The
-  // store epilogues need those 16 registers at problem ends (holding them
-  // triples stage 1's spills), so they read a chunk's fragments at its top.
-  constexpr bool kAhead = EPI == EPI_INTENSITY;
+  // Chunk s + 1's first fragments are read under chunk s's last product
+  // group, across the chunk's barrier (the complex store, whose epilogue
+  // needs those 16 registers at problem ends, reads them at the chunk's top).
+  constexpr bool kAhead = EPI != EPI_STORE_CPLX;
   // Block 0's A hi and twiddle lo fragments of chunk s.
   bf16x8_t ah[2] = {}, bl[2] = {};
   // Nothing is outstanding on the LDS/scalar counter after this wait; the
