@@ -285,7 +285,9 @@ int mftx_zero(mftx_ctx* ctx, void* p, int64_t bytes);
  * overlapping Fields (field.py:413-461).  The host computes the per-Field
  * geometry of propagate_dft and passes one mftx_problem per (tile, wavelength)
  * that reaches the output; the plan runs every problem in a few grouped
- * launches and accumulates weight*|E|^2 onto the detector image.
+ * launches and accumulates weight*|E|^2 onto the detector image.  A tile may
+ * also appear at one wavelength in several problems (different placements
+ * or shifts); each distinct output grid then gets an operand of its own.
  * ------------------------------------------------------------------------- */
 
 typedef struct {

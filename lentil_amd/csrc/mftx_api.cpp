@@ -954,7 +954,8 @@ int mftx_dft2_many(mftx_ctx* ctx, const mftx_mft* mfts, int32_t count, int32_t p
   return guarded(ctx, [&]() -> int {
   if (!ctx) return MFTX_EHANDLE;
   const int rc = check_mfts(ctx, mfts, count, precision);
-  if (rc) return rc;  if (count == 0) return MFTX_OK;
+  if (rc) return rc;
+  if (count == 0) return MFTX_OK;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   return dft2_many_run(ctx, mfts, count, precision);
   });
@@ -1318,7 +1319,9 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   for (int i = 0; i < nprob; ++i) {
     const mftx_problem& P = problems[i];
     if (P.tile < 0 || P.tile >= ntiles || P.wave < 0 || P.wave >= nwave || P.M <= 0 || P.N <= 0 ||
-        P.M > (1 << 20) || P.N > (1 << 20))
+        P.M > (1 << 20) || P.N > (1 << 20) || !std::isfinite(P.alpha_r) || !std::isfinite(P.alpha_c) ||
+        !std::isfinite(P.shift_r) || !std::isfinite(P.shift_c) || !std::isfinite(P.offset_r) ||
+        !std::isfinite(P.offset_c) || !std::isfinite(P.weight))
       return fail(ctx, MFTX_EINVAL, "bad problem " + std::to_string(i));
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1359,16 +1362,37 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     c.col_first = col_first < row_first;
   }
 
+  // Each problem's stage-1 operand is its tile's phasor at its wavelength,
+  // carrying the folds of its class's two output grids (stage 1's along K,
+  // stage 2's along rows; the grids follow the shift and size, x0 the Field
+  // offset).  Problems of one tile and orientation share an operand slot
+  // when wavelength and folds agree, and get one each when they do not (the
+  // same tile placed twice at one wavelength).
+  using Slot = std::tuple<int, double, double, double, double>;  // wave, fold1, x01, fold2, x02
+  std::vector<Slot> pslot(nprob);
+  for (auto& c : cls) {
+    const ClassKey& K = c.key;
+    const bool cf = c.col_first;
+    for (int i : c.members) {
+      const mftx_problem& P = problems[i];
+      const mftx_tile& T = tiles[P.tile];
+      const OutGrid g1 = make_grid(-(cf ? P.alpha_r : P.alpha_c), cf ? K.sr : K.sc, cf ? K.M : K.N);
+      const OutGrid g2 = make_grid(-(cf ? K.ac : K.ar), cf ? K.sc : K.sr, cf ? K.N : K.M);
+      pslot[i] = Slot{P.wave, g1.fold, cf ? -centre(T.m) + P.offset_r : -centre(T.n) + P.offset_c, g2.fold,
+                      cf ? -centre(T.n) + P.offset_c : -centre(T.m) + P.offset_r};
+    }
+  }
+
   // ---- workspace layout ----------------------------------------------------------
   Layout lay;
-  // phasor operands per (tile, orientation): every wavelength the pair is used with
-  std::map<std::pair<int, int>, std::vector<int>> tile_waves;
+  // phasor operands per (tile, orientation): every slot the pair is used with
+  std::map<std::pair<int, int>, std::vector<Slot>> tile_slots;
   for (auto& c : cls)
-    for (int i : c.members) tile_waves[{problems[i].tile, c.col_first ? 1 : 0}].push_back(problems[i].wave);
-  struct TileOp { int64_t rows; int kpad; size_t off; std::map<int, int> slot; int wave_off; std::vector<int> waves; };
+    for (int i : c.members) tile_slots[{problems[i].tile, c.col_first ? 1 : 0}].push_back(pslot[i]);
+  struct TileOp { int64_t rows; int kpad; size_t off; std::map<Slot, int> slot; int wave_off; int nslot; };
   std::map<std::pair<int, int>, TileOp> tops;
-  std::vector<std::tuple<int, int, int>> wave_slots;  // (tile, orientation, wave) per phasor slot
-  for (auto& kv : tile_waves) {
+  std::vector<Slot> wave_slots;  // every phasor slot, grouped by (tile, orientation)
+  for (auto& kv : tile_slots) {
     auto v = kv.second;
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
@@ -1378,10 +1402,10 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     op.rows = round_up(tr ? T.n : T.m, BM);
     op.kpad = (int)round_up(tr ? T.m : T.n, kg);
     op.wave_off = (int)wave_slots.size();
-    op.waves = v;
+    op.nslot = (int)v.size();
     for (size_t j = 0; j < v.size(); ++j) {
       op.slot[v[j]] = (int)j;
-      wave_slots.emplace_back(kv.first.first, kv.first.second, v[j]);
+      wave_slots.push_back(v[j]);
     }
     op.off = lay.take(packed_bytes(prec, op.rows, op.kpad) * v.size());
     tops[kv.first] = op;
@@ -1402,8 +1426,6 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
 
   // ---- descriptors ---------------------------------------------------------------
   std::vector<Problem> p1, p2;
-  // (tile, orientation, wave) -> the operand's folds: stage 1 (fold, x0) along K, stage 2 (fold, x0) along rows
-  std::map<std::tuple<int, int, int>, std::array<double, 4>> stage1_fold;
   std::vector<double> cx;
   std::vector<std::vector<Work>> clus1, clus2;
   struct Patch { int idx; size_t a, dst; };  // workspace offsets to rebase after allocation
@@ -1419,14 +1441,12 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       const mftx_tile& T = tiles[P.tile];
       const TileOp& op = tops[{P.tile, cf ? 1 : 0}];
       const int kp2 = (int)round_up(cf ? T.n : T.m, kg);
-      const int n1 = cf ? K.M : K.N, n2 = cf ? K.N : K.M;  // output columns of stages 1 and 2
-      const double x01 = cf ? -centre(T.m) + P.offset_r : -centre(T.n) + P.offset_c;
-      const double x02 = cf ? -centre(T.n) + P.offset_c : -centre(T.m) + P.offset_r;
+      const int n1 = cf ? K.M : K.N;  // output columns of stage 1
+      const Slot& sl = pslot[c.members[j]];
       Problem q = {};
       q.lda = packed_lda(prec, op.kpad);
       q.alpha = -(cf ? P.alpha_r : P.alpha_c);
       const OutGrid g1 = make_grid(q.alpha, cf ? K.sr : K.sc, n1);
-      const OutGrid g2 = make_grid(-(cf ? K.ac : K.ar), cf ? K.sc : K.sr, n2);
       q.y0 = g1.y0;
       q.rows = (int)(cf ? T.n : T.m);
       q.cols = g1.slots;
@@ -1435,14 +1455,14 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       q.cm = g1.cm;
       q.kpad = op.kpad;
       q.chunk_begin = (int)cx.size();
-      push_chunks(cx, op.kpad, x01);
+      push_chunks(cx, op.kpad, std::get<2>(sl));  // x01
       q.scale = 1.0f;
       q.limit = kp2;
       q.ldd = packed_lda(prec, c.ktot);
-      // the stage-2 fold for this member's K segment; the stage-1 fold goes to the phasor
-      stage1_fold[{P.tile, cf ? 1 : 0, P.wave}] = {g1.fold, x01, g2.fold, x02};
+      // both folds are in the operand
       const size_t elem = prec == PREC_FP32 ? sizeof(float2) : 4 * sizeof(uint16_t);
-      patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)op.slot.at(P.wave) * packed_bytes(prec, op.rows, op.kpad),
+      const int slot = op.slot.at(sl);
+      patch1.push_back(Patch{(int)p1.size(), op.off + (size_t)slot * packed_bytes(prec, op.rows, op.kpad),
                              c.abuf + (size_t)c.kbeg[j] * elem});
       s1rows.push_back(op.rows);
       p1.push_back(q);
@@ -1629,8 +1649,9 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   for (size_t c = 0; c < cls.size(); ++c) plan->pf[c].a = base + patch2[cls[c].p2].a;
   std::vector<PhasorWave> iw(wave_slots.size());
   for (size_t i = 0; i < wave_slots.size(); ++i) {
-    const auto& fx = stage1_fold.at(wave_slots[i]);
-    iw[i] = PhasorWave{1.0 / wavelengths[std::get<2>(wave_slots[i])], fx[0], fx[1], fx[2], fx[3]};
+    const Slot& sl = wave_slots[i];
+    iw[i] = PhasorWave{1.0 / wavelengths[std::get<0>(sl)], std::get<1>(sl), std::get<2>(sl), std::get<3>(sl),
+                       std::get<4>(sl)};
   }
 
   plan->works1 = (Work*)(base + o_w1);
@@ -1674,7 +1695,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     ph.m = (int)T.m;
     ph.n = (int)T.n;
     ph.trans = kv.first.second;
-    ph.nwave = (int)op.waves.size();
+    ph.nwave = op.nslot;
     ph.waves = plan->waves + op.wave_off;
     ph.dst = base + op.off;
     ph.wave_stride = (long long)packed_bytes(prec, op.rows, op.kpad);

@@ -301,6 +301,80 @@ def test_model_images_are_fresh_tensors(L):
     model.close()
 
 
+# ------------------------------------------------------------------ one tile in several problems
+
+def _raw_plan_image(tile_arrays, wl, specs, image_shape, precision, lambda_group=0):
+    """A plan built straight through mftx_plan_psf (one problem per spec) and
+    executed once: the image as float64 numpy."""
+    from lentil_amd import _lib
+    lib, ctx = _lib.load(), _lib.context()
+    keep = []
+    tiles = []
+    for amp, opd in tile_arrays:
+        ta = torch.tensor(amp, dtype=torch.float32, device='cuda')
+        to = torch.tensor(opd, dtype=torch.float64, device='cuda')
+        keep += [ta, to]
+        tiles.append(_lib.MftxTile(ta.data_ptr(), to.data_ptr(), _lib.DT_F32, _lib.DT_F64, ta.stride(0), to.stride(0),
+                                   amp.shape[0], amp.shape[1]))
+    probs = [_lib.MftxProblem(s['tile'], s['wave'], s['group'], s['M'], s['N'], s['alpha'], s['alpha'], s['shift'][0],
+                              s['shift'][1], s['offset'][0], s['offset'][1], s['out'][0], s['out'][1], s['weight'])
+             for s in specs]
+    tarr = (_lib.MftxTile * len(tiles))(*tiles)
+    parr = (_lib.MftxProblem * len(probs))(*probs)
+    warr = (ctypes.c_double * len(wl))(*wl)
+    h = ctypes.c_void_p()
+    _lib.check(lib.mftx_plan_psf(ctx, tarr, len(tiles), warr, len(wl), parr, len(probs), image_shape[0],
+                                 image_shape[1], _lib.precision_code(precision), lambda_group, 0, ctypes.byref(h)),
+               ctx, 'mftx_plan_psf')
+    img = torch.full(image_shape, -1.0, dtype=torch.float32, device='cuda')
+    try:
+        _lib.check(lib.mftx_plan_execute(h, ctypes.c_void_p(img.data_ptr()), _lib.DT_F32, image_shape[1], 0), ctx,
+                   'mftx_plan_execute')
+        torch.cuda.synchronize()
+    finally:
+        lib.mftx_plan_destroy(h)
+    return img.double().cpu().numpy()
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_plan_one_tile_twice_at_one_wavelength(L, precision):
+    """The C ABI lets one tile appear at one wavelength in several problems
+    with different output grids (here one pupil placed twice on the detector
+    with different sub-pixel shifts and Field offsets, one placement clipped
+    by the image edge, plus a third problem at another wavelength).  Each
+    placement needs an operand carrying its own grid's folds: the image is
+    the weighted sum of |dft2|^2 of every problem (complex128 oracle), and
+    equals the sum of one-problem plans."""
+    from oracle import mft
+    rng = np.random.default_rng(5)
+    m = 48
+    amp = inputs.circle((m, m), 22)
+    opd = inputs.zernike_compose(amp, np.r_[0, rng.normal(0, 60e-9, size=8)])
+    wl = [650e-9, 540e-9]
+    alpha = 1.0 / (2 * m)
+    specs = [dict(tile=0, wave=0, group=0, M=64, N=64, alpha=alpha, shift=(0.3, -0.2), offset=(0.0, 0.0),
+                  out=(4, 6), weight=1.0),
+             dict(tile=0, wave=0, group=1, M=64, N=64, alpha=alpha, shift=(-0.45, 0.15), offset=(1.5, -2.0),
+                  out=(40, -10), weight=0.7),
+             dict(tile=0, wave=1, group=2, M=64, N=64, alpha=alpha * 650 / 540, shift=(0.1, 0.4), offset=(0.0, 0.0),
+                  out=(20, 30), weight=0.5)]
+    shape = (100, 90)
+    ref = np.zeros(shape)
+    for s in specs:
+        f = amp * np.exp(2j * np.pi * opd / wl[s['wave']])
+        F = mft.dft2(f, alpha=s['alpha'], shape=(s['M'], s['N']), shift=s['shift'], offset=s['offset'])
+        r0, c0 = s['out']
+        rs, cs = slice(max(r0, 0), min(r0 + s['M'], shape[0])), slice(max(c0, 0), min(c0 + s['N'], shape[1]))
+        ref[rs, cs] += s['weight'] * np.abs(F[rs.start - r0:rs.stop - r0, cs.start - c0:cs.stop - c0]) ** 2
+    print(f'one tile twice: {precision}', flush=True)
+    img = _raw_plan_image([(amp, opd)], wl, specs, shape, precision)
+    err = rel_l2(img, ref)
+    assert err < TOL_I, err
+    singles = sum(_raw_plan_image([(amp, opd)], wl, [s], shape, precision) for s in specs)
+    err = rel_l2(img, singles)
+    assert err < 1e-6, err
+
+
 # ------------------------------------------------------------------ error paths with a device
 
 def test_status_codes_with_a_context(L):
