@@ -12,7 +12,11 @@ from . import _lib
 
 
 def is_dev(x):
-    return isinstance(x, torch.Tensor)
+    """A torch tensor in the library's device memory.  Anything else (NumPy
+    arrays, CPU tensors, tensors on another device) is host-side data that is
+    copied to the device where a kernel reads it, so no kernel is ever handed
+    a pointer it cannot dereference."""
+    return isinstance(x, torch.Tensor) and x.is_cuda and x.device == _lib.device()
 
 
 def shape_of(x):
@@ -26,7 +30,7 @@ def size_of(x):
 def to_dev(x, real=False):
     """Device tensor of x: complex128/float64 stay exact; the kernels read them directly."""
     dev = _lib.device()
-    if is_dev(x):
+    if isinstance(x, torch.Tensor):
         t = x.to(dev)
     else:
         a = np.asarray(x)
@@ -43,6 +47,15 @@ def to_dev(x, real=False):
     if t.dim() == 2 and t.stride(1) != 1:
         t = t.contiguous()
     return t
+
+
+def write_back(dst, src):
+    """Copy a device tensor into a host-side destination (a NumPy array or a
+    view of one, or a CPU or other-device torch tensor), in place."""
+    if isinstance(dst, torch.Tensor):
+        dst.copy_(src)
+    else:
+        dst[...] = src.cpu().numpy()
 
 
 def to_host(x, dtype=complex):

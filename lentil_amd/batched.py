@@ -420,8 +420,9 @@ class PSFPlan:
         if out is None:
             out = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
             accumulate = False
-        if tuple(out.shape) != self.shape_out or out.dim() != 2 or out.stride(1) != 1:
-            raise ValueError(f'out must be a {self.shape_out} row-major device array')
+        if (tuple(out.shape) != self.shape_out or out.dim() != 2 or out.stride(1) != 1
+                or out.device != _lib.device()):
+            raise ValueError(f'out must be a {self.shape_out} row-major array on {_lib.device()}')
         code = _lib.dtype_code(out)
         if code not in (_lib.DT_F32, _lib.DT_F64):
             raise TypeError('out must be float32 or float64')
@@ -440,6 +441,9 @@ class PSFPlan:
         events between eagerly queued groups.  ``mode`` says which was used."""
         if out is None:
             out = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
+        if (tuple(out.shape) != self.shape_out or out.dim() != 2 or out.stride(1) != 1
+                or out.device != _lib.device()):
+            raise ValueError(f'out must be a {self.shape_out} row-major array on {_lib.device()}')
         ms = (ctypes.c_double * 5)()
         ctx = _lib.context()
         rc = _lib.load().mftx_plan_profile(self._plan, ctypes.c_void_p(out.data_ptr()), _lib.dtype_code(out),
@@ -477,7 +481,7 @@ class PSFPlan:
             raise ValueError(f'set_fields needs one entry per problem ({n})')
         ld = dst[0].stride(0)
         for t, p in zip(dst, self.problems):
-            if (t.dtype != torch.complex64 or t.device.type != 'cuda' or tuple(t.shape) != (p['M'], p['N'])
+            if (t.dtype != torch.complex64 or t.device != _lib.device() or tuple(t.shape) != (p['M'], p['N'])
                     or t.stride(1) != 1 or t.stride(0) != ld):
                 raise ValueError('fields must be M x N complex64 device tensors with one row stride')
         wdt, ldw = _lib.DT_F32, 0
@@ -485,7 +489,7 @@ class PSFPlan:
             wdt = _lib.dtype_code(weights[0])
             ldw = weights[0].stride(0)
             for t, p in zip(weights, self.problems):
-                if (_lib.dtype_code(t) != wdt or wdt not in (_lib.DT_F32, _lib.DT_F64) or t.device.type != 'cuda'
+                if (_lib.dtype_code(t) != wdt or wdt not in (_lib.DT_F32, _lib.DT_F64) or t.device != _lib.device()
                         or tuple(t.shape) != (p['M'], p['N']) or t.stride(1) != 1 or t.stride(0) != ldw):
                     raise ValueError('weight maps must be M x N real device tensors with one row stride')
         self._fields_keep = (list(dst), None if weights is None else list(weights))

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib, extent as _ext
-from ._arrays import is_dev, shape_of, size_of, to_dev, to_host, row_stride
+from ._arrays import is_dev, shape_of, size_of, to_dev, to_host, row_stride, write_back
 
 
 class Field:
@@ -225,7 +225,7 @@ def insert(field, out, intensity=False, weight=1):
     out_arr = np.asarray(out)
     dev_out = to_dev(out_arr)
     _device_insert(field.data, fr0, fc0, h, w, dev_out, or0, oc0, intensity, weight)
-    out_arr[...] = dev_out.cpu().numpy()
+    write_back(out_arr, dev_out)
     return out
 
 

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._arrays import is_dev, to_dev
+from ._arrays import is_dev, to_dev, write_back
 
 _default_precision = 'fp32'
 
@@ -46,7 +46,7 @@ def _mft(f, alpha, shape, shift, offset, scale, conj, out, precision):
     off_r, off_c = (float(v) for v in np.broadcast_to(offset, (2,)))
     if out is not None:
         odt = out.dtype
-        ok = odt.is_complex if is_dev(out) else np.can_cast(complex, odt)
+        ok = odt.is_complex if isinstance(out, torch.Tensor) else np.can_cast(complex, odt)
         if not ok:
             raise TypeError(f"Cannot cast complex output to dtype('{odt}')")
     prec = _lib.precision_code(precision or _default_precision)
@@ -71,7 +71,7 @@ def _mft(f, alpha, shape, shift, offset, scale, conj, out, precision):
     if is_dev(out):
         out.copy_(F)
     else:
-        out[...] = F.cpu().numpy()
+        write_back(out, F)
     return out
 
 

@@ -15,7 +15,7 @@ import torch
 
 from . import _lib, extent as _ext, field as _field, fourier
 from .ptype import pupil as PT_PUPIL, image as PT_IMAGE
-from ._arrays import to_dev
+from ._arrays import is_dev, to_dev, write_back
 from .field import Field
 from .util import boundary
 from .wavefront import Wavefront
@@ -174,7 +174,7 @@ def propagate_fft(wavefront, pixelscale, shape=None, oversample=2, scratch=None,
     if scratch is not None:
         if not all(np.asarray(scratch.shape) > n):
             raise ValueError(f'scratch must have shape greater than or equal to {tuple(n)}')
-        on_device = isinstance(scratch, torch.Tensor) and scratch.is_cuda
+        on_device = is_dev(scratch)
         if on_device:
             buf = scratch[0:n0, 0:n1]
             buf.zero_()
@@ -183,7 +183,7 @@ def propagate_fft(wavefront, pixelscale, shape=None, oversample=2, scratch=None,
         for f in wavefront.data:
             _field.insert(f, buf)
         if not on_device:
-            scratch[0:n0, 0:n1] = buf.cpu().numpy()
+            write_back(scratch[0:n0, 0:n1], buf)
         src, pad_off = buf, (0, 0)
     else:
         full = wavefront.field_tensor()

@@ -4,7 +4,7 @@ import torch
 
 from . import _lib, field as _field
 from .ptype import ptype as as_ptype, none as PT_NONE, pupil as PT_PUPIL, image as PT_IMAGE
-from ._arrays import is_dev
+from ._arrays import is_dev, write_back
 from .field import Field
 
 
@@ -107,5 +107,5 @@ class Wavefront:
         dev = torch.from_numpy(np.ascontiguousarray(out)).to(_lib.device())
         for f in fields:
             _field.insert(f, dev, intensity=True, weight=weight)
-        out[...] = dev.cpu().numpy()
+        write_back(out, dev)
         return out

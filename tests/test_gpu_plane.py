@@ -280,6 +280,49 @@ def test_plan_writes_only_its_view(L):
         plan.close()
 
 
+def test_host_tensors_take_the_host_path(L):
+    """CPU torch tensors are host data, like NumPy arrays: an input is copied
+    to the device and an output is written back, never handed to a kernel
+    as a pointer; outputs a kernel writes directly must be on the device."""
+    from oracle import mft
+    rng = np.random.default_rng(3)
+    f = rng.normal(size=(12, 10)) + 1j * rng.normal(size=(12, 10))
+    kw = dict(shape=(16, 14), shift=(0.3, -0.1))
+    ref = mft.dft2(f, 0.07, **kw)
+    got = L.fourier.dft2(torch.tensor(f), 0.07, **kw)
+    assert isinstance(got, np.ndarray)
+    err = rel_l2(got, ref)
+    assert err < 1e-5, err
+    o = torch.zeros((16, 14), dtype=torch.complex128)
+    assert L.fourier.dft2(f, 0.07, out=o, **kw) is o
+    err = rel_l2(o.numpy(), ref)
+    assert err < 1e-5, err
+
+    n = 32
+    amp = inputs.circle((n, n), n // 2 - 1)
+    opd = inputs.zernike_compose(amp, [0, 0, 0, 60e-9])
+    pupil = L.Pupil(amplitude=amp, opd=opd, pixelscale=1 / n, focal_length=10)
+    w = L.propagate_dft(L.Wavefront(650e-9) * pupil, 5e-6, shape=16, oversample=2)
+    host = w.insert(np.zeros((32, 32)), weight=0.5)
+    cpu_t = torch.zeros((32, 32), dtype=torch.float64)
+    assert w.insert(cpu_t, weight=0.5) is cpu_t
+    assert np.array_equal(cpu_t.numpy(), host)
+
+    plan = L.PSFPlan(pupil, [650e-9], [1.0], pixelscale=5e-6, shape=16, oversample=2)
+    with pytest.raises(ValueError):
+        plan(torch.zeros(plan.shape_out, dtype=torch.float32))
+    with pytest.raises(ValueError):
+        plan.profile(torch.zeros(plan.shape_out, dtype=torch.float32))
+    with pytest.raises(ValueError):
+        plan.set_fields([torch.zeros((p['M'], p['N']), dtype=torch.complex64) for p in plan.problems],
+                        [1.0] * plan.n_problems)
+    img = plan()
+    torch.cuda.synchronize()
+    err = rel_l2(img.double().cpu().numpy(), host * 2)
+    assert err < 1e-5, err
+    plan.close()
+
+
 def test_model_images_are_fresh_tensors(L):
     """With one diversity image the model still returns a tensor of its own:
     the next forward call does not overwrite an image already returned."""
@@ -407,6 +450,18 @@ def test_status_codes_with_a_context(L):
     out = L.fourier.dft2(np.ones((8, 8)), 0.125)
     ref = optics_dft2(np.ones((8, 8)), 0.125)
     assert rel_l2(out, ref) < 1e-6
+    # a plan problem with non-finite geometry is refused before anything is built
+    amp = torch.ones((8, 8), dtype=torch.float32, device='cuda')
+    opd = torch.zeros((8, 8), dtype=torch.float64, device='cuda')
+    tiles = (_lib.MftxTile * 1)(_lib.MftxTile(amp.data_ptr(), opd.data_ptr(), _lib.DT_F32, _lib.DT_F64, 8, 8, 8, 8))
+    warr = (ctypes.c_double * 1)(600e-9)
+    for field in ('alpha_r', 'shift_c', 'offset_r', 'weight'):
+        p = _lib.MftxProblem(0, 0, 0, 16, 16, 0.05, 0.05, 0.0, 0.0, 0.0, 0.0, 0, 0, 1.0)
+        setattr(p, field, float('nan') if field != 'weight' else float('inf'))
+        h = ctypes.c_void_p()
+        rc = lib.mftx_plan_psf(ctx, tiles, 1, warr, 1, (_lib.MftxProblem * 1)(p), 1, 16, 16, 0, 0, 0, ctypes.byref(h))
+        assert rc == _lib.EINVAL and h.value is None, field
+        assert b'bad problem' in lib.mftx_last_error(ctx), field
 
 
 def optics_dft2(f, alpha):
