@@ -110,7 +110,8 @@ int mftx_phasor(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp,
 /*
  * out[or0 + r][oc0 + c] += weight * |E|^2   (intensity != 0, out float32/float64)
  * out[or0 + r][oc0 + c] += weight * E       (intensity == 0, out complex64/complex128)
- * for E = field[fr0 + r][fc0 + c], 0 <= r < h, 0 <= c < w.
+ * for E = field[fr0 + r][fc0 + c], 0 <= r < h, 0 <= c < w (both slices within
+ * their row strides: fc0 + w <= ldf, oc0 + w <= ldo; h, w <= 2^20).
  * Replaces the accumulation of field.insert (field.py:301-304); the caller
  * passes the clipped slices of field.py:259-299.  Coherent merging of
  * overlapping Fields (field._merge, field.py:339-342) is the complex form
@@ -203,7 +204,8 @@ int mftx_mft_plan_destroy(mftx_mft_plan* plan);
 
 /* Integer block sums of nimg stacked rows x cols images (image i at
  * img + i * rows * ld) into nimg contiguous (rows/factor) x (cols/factor)
- * outputs: util.rebin (util.py:221-259). */
+ * outputs: util.rebin (util.py:221-259).  factor must divide rows and cols
+ * (MFTX_EINVAL otherwise, as the reference's reshape fails). */
 int mftx_rebin(mftx_ctx* ctx, const void* img, int dtype, int64_t nimg, int64_t rows, int64_t cols, int64_t ld,
                int64_t factor, void* out, int out_dtype);
 

@@ -863,7 +863,8 @@ int mftx_phasor(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_amp, c
   if (!amp || !opd || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
   if ((amp_dtype != 1 && amp_dtype != 3) || (opd_dtype != 1 && opd_dtype != 3))
     return fail(ctx, MFTX_EINVAL, "amp/opd must be float32 or float64");
-  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n) return fail(ctx, MFTX_EINVAL, "bad phasor shape");
+  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n || m > (1 << 20) || n > (1 << 20))
+    return fail(ctx, MFTX_EINVAL, "bad phasor shape");
   if (!(wavelength > 0)) return fail(ctx, MFTX_EINVAL, "wavelength must be positive");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, launch_phasor_field(amp, amp_dtype, ld_amp, opd, opd_dtype, ld_opd, (int)m, (int)n, 1.0 / wavelength,
@@ -876,6 +877,10 @@ int mftx_insert(mftx_ctx* ctx, const void* field, int64_t ldf, int64_t fr0, int6
   if (!ctx) return MFTX_EHANDLE;
   if (!field || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
   if (h < 0 || w < 0 || fr0 < 0 || fc0 < 0 || or0 < 0 || oc0 < 0) return fail(ctx, MFTX_EINVAL, "negative slice");
+  if (h > (1 << 20) || w > (1 << 20) || fr0 > (1 << 30) || fc0 > (1 << 30) || or0 > (1 << 30) || oc0 > (1 << 30))
+    return fail(ctx, MFTX_EINVAL, "slice too large");
+  if (h > 0 && w > 0 && (ldf < fc0 + w || ldo < oc0 + w))
+    return fail(ctx, MFTX_EINVAL, "slice wider than its row stride");
   if (intensity && out_dtype != 1 && out_dtype != 3) return fail(ctx, MFTX_EINVAL, "intensity output must be real");
   if (!intensity && out_dtype != 0 && out_dtype != 2) return fail(ctx, MFTX_EINVAL, "field output must be complex");
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -888,7 +893,8 @@ int mftx_multiply(mftx_ctx* ctx, const void* a, int64_t lda, const void* b, int6
                   int64_t h, int64_t w) {
   if (!ctx) return MFTX_EHANDLE;
   if (!a || !b || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
-  if (h < 0 || w < 0 || lda < w || ldb < w || ldo < w) return fail(ctx, MFTX_EINVAL, "bad multiply shape");
+  if (h < 0 || w < 0 || lda < w || ldb < w || ldo < w || h > (1 << 20) || w > (1 << 20))
+    return fail(ctx, MFTX_EINVAL, "bad multiply shape");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, launch_multiply(a, lda, b, ldb, out, ldo, (int)h, (int)w, ctx->stream));
   return MFTX_OK;
@@ -899,7 +905,8 @@ int mftx_weight_field(mftx_ctx* ctx, const void* F, int64_t ldF, const void* w, 
   if (!ctx) return MFTX_EHANDLE;
   if (!F || !w || !out) return fail(ctx, MFTX_EINVAL, "null array pointer");
   if (w_dtype != 1 && w_dtype != 3) return fail(ctx, MFTX_EINVAL, "weight must be float32 or float64");
-  if (h < 0 || wd < 0 || ldF < wd || ldw < wd || ldo < wd) return fail(ctx, MFTX_EINVAL, "bad weight shape");
+  if (h < 0 || wd < 0 || ldF < wd || ldw < wd || ldo < wd || h > (1 << 20) || wd > (1 << 20))
+    return fail(ctx, MFTX_EINVAL, "bad weight shape");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, launch_weight_field(F, ldF, w, w_dtype, ldw, scale, out, ldo, (int)h, (int)wd, ctx->stream));
   return MFTX_OK;
@@ -912,7 +919,8 @@ int mftx_phase_grad(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t ld_am
   if (!amp || !opd || !D || !grad_opd) return fail(ctx, MFTX_EINVAL, "null array pointer");
   if ((amp_dtype != 1 && amp_dtype != 3) || (opd_dtype != 1 && opd_dtype != 3))
     return fail(ctx, MFTX_EINVAL, "amp/opd must be float32 or float64");
-  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n || ldD < n || ld_gopd < n || (grad_amp && ld_gamp < n))
+  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n || ldD < n || ld_gopd < n || (grad_amp && ld_gamp < n) ||
+      m > (1 << 20) || n > (1 << 20))
     return fail(ctx, MFTX_EINVAL, "bad gradient shape");
   if (!(wavelength > 0)) return fail(ctx, MFTX_EINVAL, "wavelength must be positive");
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -930,7 +938,8 @@ int mftx_phase_grad_many(mftx_ctx* ctx, const void* amp, int amp_dtype, int64_t 
     return fail(ctx, MFTX_EINVAL, "null array pointer");
   if ((amp_dtype != 1 && amp_dtype != 3) || (opd_dtype != 1 && opd_dtype != 3))
     return fail(ctx, MFTX_EINVAL, "amp/opd must be float32 or float64");
-  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n || ldD < n || ld_gopd < n || (grad_amp && ld_gamp < n))
+  if (m <= 0 || n <= 0 || ld_amp < n || ld_opd < n || ldD < n || ld_gopd < n || (grad_amp && ld_gamp < n) ||
+      m > (1 << 20) || n > (1 << 20))
     return fail(ctx, MFTX_EINVAL, "bad gradient shape");
   if (nwave < 0 || (nwave > 1 && strideD < m * ldD)) return fail(ctx, MFTX_EINVAL, "bad wavelength count or stride");
   if ((size_t)nwave * sizeof(double) > kSlotBytes) return fail(ctx, MFTX_EINVAL, "too many wavelengths");
@@ -1018,6 +1027,8 @@ int mftx_rebin(mftx_ctx* ctx, const void* img, int dtype, int64_t nimg, int64_t 
     return fail(ctx, MFTX_EINVAL, "rebin is defined for float32/float64 data");
   if (nimg <= 0 || rows <= 0 || cols <= 0 || ld < cols || factor <= 0 || factor > (1 << 16))
     return fail(ctx, MFTX_EINVAL, "bad rebin shape");
+  // util.rebin reshapes into factor x factor blocks, which needs whole blocks
+  if (rows % factor || cols % factor) return fail(ctx, MFTX_EINVAL, "rebin factor must divide the image shape");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, launch_rebin(img, dtype, nimg, rows, cols, ld, (int)factor, out, out_dtype, ctx->stream));
   return MFTX_OK;

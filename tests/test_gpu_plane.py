@@ -450,6 +450,20 @@ def test_status_codes_with_a_context(L):
     out = L.fourier.dft2(np.ones((8, 8)), 0.125)
     ref = optics_dft2(np.ones((8, 8)), 0.125)
     assert rel_l2(out, ref) < 1e-6
+    # a rebin factor that does not divide the image is refused (the reference's reshape fails)
+    img = torch.ones((10, 12), dtype=torch.float64, device='cuda')
+    small = torch.empty((3, 4), dtype=torch.float64, device='cuda')
+    assert lib.mftx_rebin(ctx, ctypes.c_void_p(img.data_ptr()), _lib.DT_F64, 1, 10, 12, 12, 3,
+                          ctypes.c_void_p(small.data_ptr()), _lib.DT_F64) == _lib.EINVAL
+    assert b'divide' in lib.mftx_last_error(ctx)
+    # an insert slice wider than its source's or destination's row stride is refused
+    fld = torch.zeros((8, 8), dtype=torch.complex64, device='cuda')
+    dst = torch.zeros((8, 8), dtype=torch.float64, device='cuda')
+    ins = lambda fc0, oc0: lib.mftx_insert(ctx, ctypes.c_void_p(fld.data_ptr()), 8, 0, fc0, 4, 6,
+                                           ctypes.c_void_p(dst.data_ptr()), _lib.DT_F64, 8, 0, oc0, 1, 1.0)
+    assert ins(2, 2) == _lib.OK
+    assert ins(3, 0) == _lib.EINVAL and ins(0, 3) == _lib.EINVAL
+    assert b'row stride' in lib.mftx_last_error(ctx)
     # a plan problem with non-finite geometry is refused before anything is built
     amp = torch.ones((8, 8), dtype=torch.float32, device='cuda')
     opd = torch.zeros((8, 8), dtype=torch.float64, device='cuda')
