@@ -308,7 +308,7 @@ typedef struct {
   double alpha_r, alpha_c;     /* propagate.py:232-234 */
   double shift_r, shift_c;     /* prop_shift + subpixel shift (propagate.py:235-236) */
   double offset_r, offset_c;   /* the Field offset (propagate.py:236) */
-  int64_t out_r0, out_c0;      /* detector index of F[0][0] (may be outside the image) */
+  int64_t out_r0, out_c0;      /* detector index of F[0][0] (may be outside the image; |.| <= 2^30) */
   double weight;               /* intensity weight of the wavelength */
 } mftx_problem;
 
@@ -324,7 +324,8 @@ typedef struct {
   int32_t n_stage1, n_stage2, n_work1, n_work2;
 } mftx_plan_info;
 
-/* Build a plan; validates every problem (MFTX_EINVAL on a bad index or shape)
+/* Build a plan; validates every problem (MFTX_EINVAL on a bad index or shape;
+ * image_rows, image_cols and every M, N at most 2^20)
  * and allocates plan-owned workspace.  Tile pointers are kept and re-read on
  * every execute, so amp/opd contents may change between executes.
  * Schedule knobs (no reference counterpart; lentil_amd.PSFPlan can pick

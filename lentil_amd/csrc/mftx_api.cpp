@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1314,7 +1315,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   *out = nullptr;
   if (ntiles <= 0 || nwave <= 0 || nprob < 0) return fail(ctx, MFTX_EINVAL, "empty plan");
   if (precision != MFTX_PREC_FP32 && precision != MFTX_PREC_BF16X3) return fail(ctx, MFTX_EINVAL, "unknown precision");
-  if (image_rows <= 0 || image_cols <= 0) return fail(ctx, MFTX_EINVAL, "bad image shape");
+  if (image_rows <= 0 || image_cols <= 0 || image_rows > (1 << 20) || image_cols > (1 << 20))
+    return fail(ctx, MFTX_EINVAL, "bad image shape");
   if (lambda_group <= 0) lambda_group = 8;
   if (tile_n == 0) tile_n = BN_DEFAULT;
   if (!valid_bn(tile_n)) return fail(ctx, MFTX_EINVAL, "tile_n must be 0, 64 or 128");
@@ -1330,7 +1332,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   for (int i = 0; i < nprob; ++i) {
     const mftx_problem& P = problems[i];
     if (P.tile < 0 || P.tile >= ntiles || P.wave < 0 || P.wave >= nwave || P.M <= 0 || P.N <= 0 ||
-        P.M > (1 << 20) || P.N > (1 << 20) || !std::isfinite(P.alpha_r) || !std::isfinite(P.alpha_c) ||
+        P.M > (1 << 20) || P.N > (1 << 20) || std::llabs(P.out_r0) > (1 << 30) || std::llabs(P.out_c0) > (1 << 30) ||
+        !std::isfinite(P.alpha_r) || !std::isfinite(P.alpha_c) ||
         !std::isfinite(P.shift_r) || !std::isfinite(P.shift_c) || !std::isfinite(P.offset_r) ||
         !std::isfinite(P.offset_c) || !std::isfinite(P.weight))
       return fail(ctx, MFTX_EINVAL, "bad problem " + std::to_string(i));
@@ -2021,12 +2024,18 @@ int mftx_plan_set_fields(mftx_plan* plan, void* const* dst, int64_t ldd, const v
   return guarded(ctx, [&]() -> int {
     if (!dst || !scale) return fail(ctx, MFTX_EINVAL, "null argument");
     if (wmap && wmap_dtype != 1 && wmap_dtype != 3) return fail(ctx, MFTX_EINVAL, "weight map must be float32 or float64");
+    // every argument is checked before any descriptor changes
     for (size_t c = 0; c < plan->pf.size(); ++c) {
-      Problem& q = plan->pf[c];
+      const Problem& q = plan->pf[c];
       const int i = plan->class_first[c];
       const int64_t N = q.trans ? q.rows : q.ncols;  // the class's F is M x N, row stride ldd
       if (!dst[i]) return fail(ctx, MFTX_EINVAL, "null field pointer for problem " + std::to_string(i));
       if (ldd < N || (wmap && wmap[i] && ldw < N)) return fail(ctx, MFTX_EINVAL, "field row stride too small");
+      if (!std::isfinite(scale[i])) return fail(ctx, MFTX_EINVAL, "non-finite field scale");
+    }
+    for (size_t c = 0; c < plan->pf.size(); ++c) {
+      Problem& q = plan->pf[c];
+      const int i = plan->class_first[c];
       q.dst = dst[i];
       q.ldd = ldd;
       q.wmap = wmap ? wmap[i] : nullptr;

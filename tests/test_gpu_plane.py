@@ -469,13 +469,19 @@ def test_status_codes_with_a_context(L):
     opd = torch.zeros((8, 8), dtype=torch.float64, device='cuda')
     tiles = (_lib.MftxTile * 1)(_lib.MftxTile(amp.data_ptr(), opd.data_ptr(), _lib.DT_F32, _lib.DT_F64, 8, 8, 8, 8))
     warr = (ctypes.c_double * 1)(600e-9)
-    for field in ('alpha_r', 'shift_c', 'offset_r', 'weight'):
+    bad = [('alpha_r', float('nan')), ('shift_c', float('nan')), ('offset_r', float('nan')),
+           ('weight', float('inf')), ('out_r0', 1 << 31), ('out_c0', -(1 << 31))]
+    for field, value in bad:
         p = _lib.MftxProblem(0, 0, 0, 16, 16, 0.05, 0.05, 0.0, 0.0, 0.0, 0.0, 0, 0, 1.0)
-        setattr(p, field, float('nan') if field != 'weight' else float('inf'))
+        setattr(p, field, value)
         h = ctypes.c_void_p()
         rc = lib.mftx_plan_psf(ctx, tiles, 1, warr, 1, (_lib.MftxProblem * 1)(p), 1, 16, 16, 0, 0, 0, ctypes.byref(h))
         assert rc == _lib.EINVAL and h.value is None, field
         assert b'bad problem' in lib.mftx_last_error(ctx), field
+    p = _lib.MftxProblem(0, 0, 0, 16, 16, 0.05, 0.05, 0.0, 0.0, 0.0, 0.0, 0, 0, 1.0)
+    h = ctypes.c_void_p()
+    rc = lib.mftx_plan_psf(ctx, tiles, 1, warr, 1, (_lib.MftxProblem * 1)(p), 1, 1 << 21, 16, 0, 0, 0, ctypes.byref(h))
+    assert rc == _lib.EINVAL and b'image shape' in lib.mftx_last_error(ctx)
 
 
 def optics_dft2(f, alpha):

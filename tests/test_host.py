@@ -191,7 +191,7 @@ def test_bench_refuses_several_gpus():
 def test_null_handles_are_rejected():
     """Every entry point that takes a context, plan or MFT-plan handle first
     returns MFTX_EHANDLE for a null handle, before touching any other
-    argument or the device (SURVEY: invalid handle is host-forcible)."""
+    argument or the device, so the error path runs without a GPU."""
     if not os.path.exists(LIB):
         pytest.skip('libmftx.so not built (run __graft_entry__.build())')
     from lentil_amd import _lib
