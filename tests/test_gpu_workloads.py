@@ -109,8 +109,11 @@ def test_stress_2048_vs_oracle(L, stress, precision, contraction):
 
 
 def test_stress_2048_full_sweep(L):
-    """All 128 wavelengths from a hipGraph: replays equal the eager result and
-    the merged bf16x3 image equals the per-tile fp32 image within the gate."""
+    """All 128 wavelengths from a hipGraph: replays equal the eager result,
+    the merged bf16x3 image equals the per-tile fp32 image within the gate,
+    and its 128 x 128 window around the off-axis core equals the oracle's
+    image of all 128 wavelengths on that window (the reference loop with a
+    detector mask, propagate.py:245-260, which evaluates only those pixels)."""
     from lentil_amd import workloads
     cfg = workloads.stress_2048()
     wl, wt = cfg['wavelengths'], cfg['weights']
@@ -124,8 +127,23 @@ def test_stress_2048_full_sweep(L):
         torch.cuda.synchronize()
         assert torch.equal(out, first)
     b = _plan(L, cfg, wl, wt, 'fp32', use_graph=False, contraction='per_tile')
-    err = rel_l2(first.double().cpu().numpy(), b().double().cpu().numpy())
+    img = first.double().cpu().numpy()
+    err = rel_l2(img, b().double().cpu().numpy())
     print(f'stress_2048 full sweep: merged bf16x3 vs per-tile fp32 rel-L2 {err:.2e}', flush=True)
+    assert err < TOL_I
+    # the oracle on a window: every wavelength, only the pixels of the mask's box
+    n, w = img.shape[0], 128
+    r0, c0 = n // 2 + 37 - w // 2, n // 2 - 13 - w // 2   # the core sits at (+37.3, -12.6) px
+    mask = np.zeros(img.shape)
+    mask[r0:r0 + w, c0:c0 + w] = 1
+    print('stress_2048 full sweep: oracle window of 128 wavelengths', flush=True)
+    p = optics.Pupil(cfg['amp'], cfg['opd'], cfg['dx'], cfg['focal_length'], mask=cfg['mask'])
+    ref = optics.broadband_image(p, wl, wt, cfg['pixelscale'], cfg['shape'], cfg['oversample'], mask=mask,
+                                 tilts=[optics.Tilt(*t) for t in cfg['tilts']])
+    win = (slice(r0, r0 + w), slice(c0, c0 + w))
+    assert np.count_nonzero(ref) == w * w and np.argmax(ref[win]) == np.argmax(img[win])
+    err = rel_l2(img[win], ref[win])
+    print(f'stress_2048 full sweep: window vs oracle rel-L2 {err:.2e}', flush=True)
     assert err < TOL_I
     a.close()
     b.close()
