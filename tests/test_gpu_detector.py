@@ -56,6 +56,32 @@ def test_pixel_reference_assertions(L):
         L.detector.pixel(np.ones((10, 12)), 2)      # the reference's MTF only broadcasts on square images
 
 
+def test_detector_at_workload_grids(L):
+    """The operators at the detector grids the workloads produce: the
+    1024^2 broadband_512 image (pixelate and jitter at oversample 2) and the
+    4096^2 stress_2048 image (pixel MTF and rebin at oversample 8), from a
+    device tensor as a plan returns it, against the oracle."""
+    img = _psf(1024, 11)
+    dev = torch.tensor(img, dtype=torch.float32, device='cuda')
+    print('detector 1024^2: pixelate, jitter', flush=True)
+    got = L.detector.pixelate(dev, 2)
+    assert got.shape == (512, 512)
+    err = rel_l2(got.double().cpu().numpy(), D.pixelate(img, 2))
+    assert err < 1e-5, err
+    err = rel_l2(L.jitter(dev, 2.0, pixelscale=1, oversample=2).double().cpu().numpy(),
+                 D.jitter(img, 2.0, oversample=2))
+    assert err < 1e-5, err
+    img = _psf(4096, 12)
+    dev = torch.tensor(img, dtype=torch.float32, device='cuda')
+    print('detector 4096^2: pixel, rebin', flush=True)
+    err = rel_l2(L.detector.pixel(dev, 8).double().cpu().numpy(), D.pixel(img, 8))
+    assert err < 1e-5, err
+    got = L.rebin(dev, 8)
+    assert tuple(got.shape) == (512, 512)
+    err = rel_l2(got.double().cpu().numpy(), D.rebin(img, 8))
+    assert err < 1e-6, err
+
+
 @pytest.mark.parametrize('n', [64, 96])
 def test_pixelate(L, n):
     img = _psf(n, 7)
