@@ -499,6 +499,24 @@ __device__ __forceinline__ void store_partial(const Work& wk, const Problem& P, 
   const int jp = P.cp + s, jm = P.cm - s;
   const bool has_p = jp < P.ncols, has_m = jm >= 0 && jm < P.ncols && jm != jp;
   const int trans = P.trans;
+  if (trans) {
+    // C rows run along the partial's rows, and a lane's elements 4g .. 4g+3
+    // are 4 consecutive C rows: one aligned 16-byte store each (the partial
+    // and its row stride are multiples of 128 floats).  Rows past pcols
+    // land in the partial's padding columns, inside its allocation, which
+    // the reduction never reads.
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int row = wk.tr * BM + wr * 32 + 8 * g + 4 * (lane >> 5);
+      if (has_p && jp < wk.prows)
+        *(gptr<v4f32>)&((gptr<float>)wk.partial)[(long long)jp * wk.ldp + row] =
+            v4f32{ip[4 * g], ip[4 * g + 1], ip[4 * g + 2], ip[4 * g + 3]};
+      if (has_m && jm < wk.prows)
+        *(gptr<v4f32>)&((gptr<float>)wk.partial)[(long long)jm * wk.ldp + row] =
+            v4f32{im[4 * g], im[4 * g + 1], im[4 * g + 2], im[4 * g + 3]};
+    }
+    return;
+  }
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
     const int row = wk.tr * BM + wr * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
