@@ -340,11 +340,12 @@ def main():
     print(f'[bench] stages ms {stage_ms}', file=sys.stderr, flush=True)
     # HBM-bound field kernels: algorithmic bytes / event-timed duration
     # a few-microsecond kernel is within the noise of the prefix-graph
-    # differences; its rate is reported only above 1 us (the kernel traces in
+    # differences; its rate is reported only when the difference is above
+    # 1 us and implies no more than the HBM peak (the kernel traces in
     # profiles/ time it directly)
     def field(k):
         ms, nbytes = stage_ms[k], plan.info[f'bytes_{k}']
-        if ms < 1e-3:
+        if ms < 1e-3 or nbytes / (ms * 1e-3) > HBM_PEAK:
             return {'ms': ms, 'bytes': nbytes, 'gb_per_s': None, 'frac_of_hbm_peak': None,
                     'note': 'below the resolution of the prefix-graph differences'}
         return {'ms': ms, 'bytes': nbytes, 'gb_per_s': nbytes / (ms * 1e-3) / 1e9,
