@@ -188,6 +188,20 @@ def test_bench_refuses_several_gpus():
     assert 'one GPU' in msg['error']
 
 
+def test_bench_reports_unresolved_stages_as_none():
+    """A stage whose prefix-graph difference is below 1 us (the few-microsecond
+    reduction can difference to zero or below) is reported as None with a
+    note, never as a negative time."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location('bench_mod', os.path.join(ROOT, 'bench.py'))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    got = bench.resolved({'phasor': 0.035, 'stage1': 0.19, 'stage2': 0.31, 'reduce': -0.0008, 'mode': 'graph replay'})
+    assert got['reduce'] is None and 'note' in got and got['stage2'] == 0.31 and got['mode'] == 'graph replay'
+    got = bench.resolved({'phasor': 0.035, 'reduce': 0.0055, 'mode': 'eager'})
+    assert got == {'phasor': 0.035, 'reduce': 0.0055, 'mode': 'eager'}
+
+
 def test_null_handles_are_rejected():
     """Every entry point that takes a context, plan or MFT-plan handle first
     returns MFTX_EHANDLE for a null handle, before touching any other
