@@ -388,30 +388,35 @@ class PSFPlan:
         _lib.check(rc, ctx, 'mftx_plan_psf')
         return h.value
 
-    def _autotune(self, repeats=5):
+    def _autotune(self, repeats=8, passes=2):
         """Time each (lambda_group, tile_n) candidate eagerly on the current
-        stream and return the fastest; self.tuning keeps the timings (ms)."""
+        stream and return the fastest; self.tuning keeps the timings (ms).
+        The candidates are timed in passes of alternating order (the second
+        runs them in reverse), and a candidate's time is the mean over its
+        passes: the power-managed clock drifts while the GPU warms, and a
+        single pass in a fixed order favours the candidates timed first."""
         lib = _lib.load()
         ctx = _lib.context()
         img = torch.empty(self.shape_out, dtype=torch.float32, device=_lib.device())
-        times = {}
-        for lg, tn in _CANDIDATES:
-            h = self._build(lg, tn)
-            try:
-                lib.mftx_plan_set_graph(h, 0)
-                for _ in range(2):
-                    _lib.check(lib.mftx_plan_execute(h, ctypes.c_void_p(img.data_ptr()), _lib.DT_F32,
-                                                     img.stride(0), 0), ctx, 'mftx_plan_execute')
-                start = torch.cuda.Event(enable_timing=True)
-                stop = torch.cuda.Event(enable_timing=True)
-                start.record()
-                for _ in range(repeats):
-                    lib.mftx_plan_execute(h, ctypes.c_void_p(img.data_ptr()), _lib.DT_F32, img.stride(0), 0)
-                stop.record()
-                stop.synchronize()
-                times[(lg, tn)] = start.elapsed_time(stop) / repeats
-            finally:
-                lib.mftx_plan_destroy(h)
+        times = {c: 0.0 for c in _CANDIDATES}
+        for p in range(passes):
+            for lg, tn in (_CANDIDATES if p % 2 == 0 else _CANDIDATES[::-1]):
+                h = self._build(lg, tn)
+                try:
+                    lib.mftx_plan_set_graph(h, 0)
+                    for _ in range(2):
+                        _lib.check(lib.mftx_plan_execute(h, ctypes.c_void_p(img.data_ptr()), _lib.DT_F32,
+                                                         img.stride(0), 0), ctx, 'mftx_plan_execute')
+                    start = torch.cuda.Event(enable_timing=True)
+                    stop = torch.cuda.Event(enable_timing=True)
+                    start.record()
+                    for _ in range(repeats):
+                        lib.mftx_plan_execute(h, ctypes.c_void_p(img.data_ptr()), _lib.DT_F32, img.stride(0), 0)
+                    stop.record()
+                    stop.synchronize()
+                    times[(lg, tn)] += start.elapsed_time(stop) / repeats / passes
+                finally:
+                    lib.mftx_plan_destroy(h)
         self.tuning = times
         return min(times, key=times.get)
 
