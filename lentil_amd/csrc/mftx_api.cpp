@@ -145,7 +145,12 @@ struct mftx_plan {
   double* chunk_x = nullptr;
   PhasorWave* waves = nullptr;  // per phasor wavelength slot: 1/λ and the stage-1 fold
   int nwork1 = 0, nwork2 = 0;
-  int epi2 = EPI_INTENSITY;
+  // stage 2 of the classes whose coherent group has several output grids:
+  // their complex fields, summed by the fallbacks below (every other class
+  // runs the intensity launch above)
+  Work* works2c = nullptr;
+  Problem* probs2c = nullptr;
+  int nwork2c = 0;
   PhasorOp* phasor_ops = nullptr;  // device descriptors of the grouped phasor launch
   int n_phasor_ops = 0;
   long long phasor_threads = 0;
@@ -1361,9 +1366,16 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   }
   std::map<std::pair<int, int>, std::vector<int>> groups;  // (wave, group) -> classes
   for (size_t c = 0; c < cls.size(); ++c) groups[{cls[c].key.wave, cls[c].group}].push_back((int)c);
-  bool fast = true;
+  // A group with one class runs the intensity path (|F|^2 summed in
+  // registers); the classes of a group with several are summed as complex
+  // fields first (the fallback), and only they.
+  bool fast = true;  // every group has one class
+  std::vector<char> cfast(cls.size(), 1);
   for (auto& g : groups)
-    if (g.second.size() > 1) fast = false;
+    if (g.second.size() > 1) {
+      fast = false;
+      for (int c : g.second) cfast[c] = 0;
+    }
 
   // contraction order per class (all members share M, N)
   for (auto& c : cls) {
@@ -1435,7 +1447,7 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     }
     c.ktot = k;
     c.abuf = lay.take(packed_bytes(prec, c.rows2, c.ktot));
-    if (!fast) c.fbuf = lay.take(sizeof(float2) * (size_t)c.key.M * c.key.N);
+    if (!cfast[&c - cls.data()]) c.fbuf = lay.take(sizeof(float2) * (size_t)c.key.M * c.key.N);
   }
 
   // ---- descriptors ---------------------------------------------------------------
@@ -1502,14 +1514,15 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     }
     q.trans = cf ? 0 : 1;
     const double unit = std::sqrt(std::fabs(K.ar * K.ac));  // unitary scale (fourier.py:100-101)
-    if (fast) {
+    const bool cl_fast = cfast[&c - cls.data()];
+    if (cl_fast) {
       q.scale = (float)(unit * unit * problems[c.members[0]].weight);
     } else {
       q.scale = (float)unit;
       q.ldd = K.N;  // F[u][v]
     }
     c.p2 = (int)p2.size();
-    patch2.push_back(Patch{(int)p2.size(), c.abuf, fast ? 0 : c.fbuf});
+    patch2.push_back(Patch{(int)p2.size(), c.abuf, cl_fast ? 0 : c.fbuf});
     p2.push_back(q);
   }
 
@@ -1532,7 +1545,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->img_cols = image_cols;
 
   // stage-2 works: λ-groups of classes with one output window and orientation
-  // accumulate |F|^2 in registers (fast path); otherwise one work set per class.
+  // accumulate |F|^2 in registers (the intensity path); a class of a
+  // several-class group gets a complex-store work set of its own.
   // A work item maps every problem's slots through its first problem's output
   // grid (store_partial), so classes share a λ-group only when their stage-2
   // grids agree too: the pairing columns (cp, cm) and the slot count, which
@@ -1542,10 +1556,11 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   // that window's single reduction.
   struct GeoInfo { std::vector<std::vector<int>> lgroups; size_t partial; int64_t prow, pcol; };
   std::vector<GeoInfo> geos;
-  if (fast) {
+  {
     using Grid = std::tuple<int, int, int>;  // stage-2 (cp, cm, slots)
     std::map<std::tuple<int, int, int64_t, int64_t, int>, std::map<Grid, std::vector<int>>> geo;
     for (size_t c = 0; c < cls.size(); ++c) {
+      if (!cfast[c]) continue;
       const Problem& q = p2[cls[c].p2];
       geo[{cls[c].key.M, cls[c].key.N, cls[c].key.r0, cls[c].key.c0, cls[c].col_first ? 1 : 0}][{q.cp, q.cm, q.cols}]
           .push_back((int)c);
@@ -1562,13 +1577,13 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
       gi.partial = lay.take(sizeof(float) * gi.prow * gi.pcol * gi.lgroups.size());
       geos.push_back(gi);
     }
-  } else {
-    plan->epi2 = EPI_STORE_CPLX;
   }
-  // stage-2 problem list in launch order (λ-groups contiguous)
-  std::vector<Problem> p2_launch;
-  std::vector<Patch> patch2_launch;
-  if (fast) {
+  // stage-2 problem lists in launch order (λ-groups contiguous), intensity
+  // and complex-store
+  std::vector<Problem> p2_launch, p2c_launch;
+  std::vector<Patch> patch2_launch, patch2c_launch;
+  std::vector<std::vector<Work>> clus2c;
+  {
     for (auto& gi : geos) {
       for (size_t g = 0; g < gi.lgroups.size(); ++g) {
         const int pb = (int)p2_launch.size();
@@ -1583,15 +1598,16 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
                   (float*)(uintptr_t)poff, (long long)gi.pcol, c0.key.M, c0.key.N);
       }
     }
-  } else {
-    for (auto& c : cls) {
-      const int pb = (int)p2_launch.size();
-      p2_launch.push_back(p2[c.p2]);
-      patch2_launch.push_back(patch2[c.p2]);
-      add_tiles(clus2, pb, pb + 1, c.rows2, p2[c.p2].cols, tile_n);
+    for (size_t ci = 0; ci < cls.size(); ++ci) {
+      if (cfast[ci]) continue;
+      const ClassInfo& c = cls[ci];
+      const int pb = (int)p2c_launch.size();
+      p2c_launch.push_back(p2[c.p2]);
+      patch2c_launch.push_back(Patch{pb, patch2[c.p2].a, patch2[c.p2].dst});
+      add_tiles(clus2c, pb, pb + 1, c.rows2, p2[c.p2].cols, tile_n);
     }
   }
-  std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2);
+  std::vector<Work> w1 = xcd_order(clus1), w2 = xcd_order(clus2), w2c = xcd_order(clus2c);
   // the complex-field set: each class's stage-2 problem, one work set per class
   std::vector<std::vector<Work>> clusf;
   for (size_t c = 0; c < cls.size(); ++c) {
@@ -1612,11 +1628,15 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     for (int i = w.pbeg; i < w.pend; ++i) mf1 += per_slot_k * p1[i].kpad;
   for (auto& w : w2)
     for (int i = w.pbeg; i < w.pend; ++i) mf2 += per_slot_k * p2_launch[i].kpad;
+  for (auto& w : w2c)
+    for (int i = w.pbeg; i < w.pend; ++i) mf2 += per_slot_k * p2c_launch[i].kpad;
 
   const size_t o_w1 = lay.take(sizeof(Work) * std::max<size_t>(w1.size(), 1));
   const size_t o_w2 = lay.take(sizeof(Work) * std::max<size_t>(w2.size(), 1));
   const size_t o_p1 = lay.take(sizeof(Problem) * std::max<size_t>(p1.size(), 1));
   const size_t o_p2 = lay.take(sizeof(Problem) * std::max<size_t>(p2_launch.size(), 1));
+  const size_t o_w2c = lay.take(sizeof(Work) * std::max<size_t>(w2c.size(), 1));
+  const size_t o_p2c = lay.take(sizeof(Problem) * std::max<size_t>(p2c_launch.size(), 1));
   const size_t o_cx = lay.take(sizeof(double) * std::max<size_t>(cx.size(), 1));
   const size_t o_iw = lay.take(sizeof(PhasorWave) * std::max<size_t>(wave_slots.size(), 1));
   const size_t o_ops = lay.take(sizeof(PhasorOp) * std::max<size_t>(tops.size(), 1));
@@ -1625,11 +1645,12 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   // partial-overlap groups: each group's union box on the image (its Fields'
   // placements clipped to the image, field.py:282-296); one accumulator of
   // the largest box serves every group in turn (same stream, in order)
-  std::vector<std::array<int64_t, 4>> fb_box;  // r0, c0, rows, cols per group (groups' map order)
+  std::vector<std::array<int64_t, 4>> fb_box;  // r0, c0, rows, cols per several-class group (map order)
   size_t fb_acc = 0;
   if (!fast) {
     int64_t most = 0;
     for (auto& kv : groups) {
+      if (kv.second.size() < 2) continue;
       int64_t r0 = image_rows, c0 = image_cols, r1 = 0, c1 = 0;
       for (int c : kv.second) {
         const ClassKey& K = cls[c].key;
@@ -1658,6 +1679,10 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
     p2_launch[pt.idx].a = base + pt.a;
     p2_launch[pt.idx].dst = pt.dst ? (void*)(base + pt.dst) : nullptr;
   }
+  for (auto& pt : patch2c_launch) {
+    p2c_launch[pt.idx].a = base + pt.a;
+    p2c_launch[pt.idx].dst = base + pt.dst;
+  }
   for (auto& w : w2)
     if (w.partial) w.partial = (float*)(base + (size_t)(uintptr_t)w.partial);
   for (size_t c = 0; c < cls.size(); ++c) plan->pf[c].a = base + patch2[cls[c].p2].a;
@@ -1676,6 +1701,9 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->waves = (PhasorWave*)(base + o_iw);
   plan->nwork1 = (int)w1.size();
   plan->nwork2 = (int)w2.size();
+  plan->works2c = (Work*)(base + o_w2c);
+  plan->probs2c = (Problem*)(base + o_p2c);
+  plan->nwork2c = (int)w2c.size();
   plan->works_f = (Work*)(base + o_wf);
   plan->probs_f = (Problem*)(base + o_pf);
   plan->nwork_f = (int)wf.size();
@@ -1687,6 +1715,8 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   up(plan->works2, w2.data(), sizeof(Work) * w2.size());
   up(plan->probs1, p1.data(), sizeof(Problem) * p1.size());
   up(plan->probs2, p2_launch.data(), sizeof(Problem) * p2_launch.size());
+  up(plan->works2c, w2c.data(), sizeof(Work) * w2c.size());
+  up(plan->probs2c, p2c_launch.data(), sizeof(Problem) * p2c_launch.size());
   up(plan->chunk_x, cx.data(), sizeof(double) * cx.size());
   up(plan->waves, iw.data(), sizeof(PhasorWave) * iw.size());
   up(plan->works_f, wf.data(), sizeof(Work) * wf.size());
@@ -1726,16 +1756,16 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   if (e != hipSuccess) {
     return hipfail(ctx, e, "plan descriptor upload");
   }
-  if (fast) {
-    for (auto& gi : geos) {
-      const ClassInfo& c0 = cls[gi.lgroups[0][0]];
-      plan->reduces.push_back(mftx_plan::Reduce{(float*)(base + gi.partial), (long long)(gi.prow * gi.pcol),
-                                                (long long)gi.pcol, (int)gi.lgroups.size(), c0.key.M, c0.key.N,
-                                                c0.key.r0, c0.key.c0});
-    }
-  } else {
+  for (auto& gi : geos) {
+    const ClassInfo& c0 = cls[gi.lgroups[0][0]];
+    plan->reduces.push_back(mftx_plan::Reduce{(float*)(base + gi.partial), (long long)(gi.prow * gi.pcol),
+                                              (long long)gi.pcol, (int)gi.lgroups.size(), c0.key.M, c0.key.N,
+                                              c0.key.r0, c0.key.c0});
+  }
+  {
     size_t g = 0;
     for (auto& kv : groups) {
+      if (kv.second.size() < 2) continue;
       mftx_plan::Fallback fb;
       fb.acc = (float2*)(base + fb_acc);
       fb.br0 = fb_box[g][0];
@@ -1787,8 +1817,10 @@ int mftx_plan_psf(mftx_ctx* ctx, const mftx_tile* tiles, int32_t ntiles, const d
   plan->info.n_stage1 = (int)p1.size();
   plan->info.n_stage2 = (int)cls.size();
   plan->info.n_work1 = plan->nwork1;
-  plan->info.n_work2 = plan->nwork2;
-  plan->info.n_launch = (ops.empty() ? 0 : 1) + 2 + (int)plan->reduces.size();
+  plan->info.n_work2 = plan->nwork2 + plan->nwork2c;
+  int nl = (ops.empty() ? 0 : 1) + 1 + (w2.empty() ? 0 : 1) + (w2c.empty() ? 0 : 1) + (int)plan->reduces.size();
+  for (auto& fb : plan->fallbacks) nl += 2 + (int)fb.adds.size();  // clear, adds, |.|^2 into the image
+  plan->info.n_launch = nl;
   *out = owned.release();
   return MFTX_OK;
   });
@@ -1845,14 +1877,39 @@ static hipError_t launch_fallbacks(const mftx_plan* plan, void* image, int dtype
   return e;
 }
 
+// Stage 2: the intensity launch (classes alone in their coherent group) and
+// the complex-store launch (the classes of several-class groups).
+static hipError_t launch_stage2(const mftx_plan* plan, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (plan->nwork2)
+    e = launch_gemm(plan->prec, EPI_INTENSITY, plan->bn, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s);
+  if (e == hipSuccess && plan->nwork2c)
+    e = launch_gemm(plan->prec, EPI_STORE_CPLX, plan->bn, plan->works2c, plan->nwork2c, plan->probs2c, plan->chunk_x,
+                    s);
+  return e;
+}
+
+// The image: the reductions write (or clear and add) it, then the
+// several-class groups add their |sum of fields|^2.
+static hipError_t launch_finish(const mftx_plan* plan, void* image, int dtype, int64_t ldi, int accumulate,
+                                hipStream_t s) {
+  if (plan->fallbacks.empty()) return launch_reduces(plan, image, dtype, ldi, accumulate, s);
+  hipError_t e = hipSuccess;
+  if (!plan->reduces.empty()) {
+    e = launch_reduces(plan, image, dtype, ldi, accumulate, s);
+    accumulate = 1;
+  }
+  if (e == hipSuccess) e = launch_fallbacks(plan, image, dtype, ldi, accumulate, s);
+  return e;
+}
+
 static int plan_launch(mftx_plan* plan, void* image, int dtype, int64_t ldi, int accumulate) {
   mftx_ctx* ctx = plan->ctx;
   hipStream_t s = ctx->stream;
   HIPCHK(ctx, launch_phasors(plan, s));
   HIPCHK(ctx, launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->bn, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x, s));
-  HIPCHK(ctx, launch_gemm(plan->prec, plan->epi2, plan->bn, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s));
-  if (plan->epi2 == EPI_INTENSITY) HIPCHK(ctx, launch_reduces(plan, image, dtype, ldi, accumulate, s));
-  else HIPCHK(ctx, launch_fallbacks(plan, image, dtype, ldi, accumulate, s));
+  HIPCHK(ctx, launch_stage2(plan, s));
+  HIPCHK(ctx, launch_finish(plan, image, dtype, ldi, accumulate, s));
   return MFTX_OK;
 }
 
@@ -1913,11 +1970,8 @@ static hipError_t profile_launches(mftx_plan* plan, void* image, int dtype, int6
     if (k == 1)
       e = launch_gemm(plan->prec, EPI_STORE_PACKED_T, plan->bn, plan->works1, plan->nwork1, plan->probs1, plan->chunk_x,
                       s);
-    if (k == 2)
-      e = launch_gemm(plan->prec, plan->epi2, plan->bn, plan->works2, plan->nwork2, plan->probs2, plan->chunk_x, s);
-    if (k == 3)
-      e = plan->epi2 == EPI_INTENSITY ? launch_reduces(plan, image, dtype, ldi, 0, s)
-                                      : launch_fallbacks(plan, image, dtype, ldi, 0, s);
+    if (k == 2) e = launch_stage2(plan, s);
+    if (k == 3) e = launch_finish(plan, image, dtype, ldi, 0, s);
     if (e == hipSuccess && ev) e = hipEventRecord(ev[k + 1], s);
   }
   return e;

@@ -418,6 +418,60 @@ def test_plan_one_tile_twice_at_one_wavelength(L, precision):
     assert err < 1e-6, err
 
 
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_plan_mixes_intensity_and_coherent_groups(L, precision):
+    """A plan whose coherent groups differ: at one wavelength two tiles summed
+    coherently with different shifts and placements (two output grids: their
+    fields are summed before |.|^2), at another a lone problem (the
+    intensity path).  The image is the oracle's: |F_a + F_b|^2 on the union
+    box plus w |F_c|^2."""
+    from oracle import mft
+    from lentil_amd import _lib
+    rng = np.random.default_rng(9)
+    m = 40
+    amp_a = inputs.circle((m, m), 18)
+    amp_b = inputs.circle((m, m), 12, shift=(3, -2))
+    opd_a = inputs.zernike_compose(amp_a, np.r_[0, rng.normal(0, 50e-9, size=6)])
+    opd_b = inputs.zernike_compose(amp_b, np.r_[0, rng.normal(0, 50e-9, size=6)])
+    wl = [600e-9, 700e-9]
+    alpha = 1.0 / (2 * m)
+    specs = [dict(tile=0, wave=0, group=0, M=48, N=48, alpha=alpha, shift=(0.3, -0.2), offset=(0.0, 0.0),
+                  out=(4, 6), weight=1.0),
+             dict(tile=1, wave=0, group=0, M=48, N=48, alpha=alpha, shift=(0.1, 0.25), offset=(1.0, -1.5),
+                  out=(10, -3), weight=1.0),
+             dict(tile=0, wave=1, group=1, M=48, N=48, alpha=alpha * 6 / 7, shift=(-0.2, 0.4), offset=(0.0, 0.0),
+                  out=(30, 40), weight=0.6)]
+    shape = (90, 80)
+    tiles = [(amp_a, opd_a), (amp_b, opd_b)]
+
+    def field(sp):
+        amp, opd = tiles[sp['tile']]
+        return mft.dft2(amp * np.exp(2j * np.pi * opd / wl[sp['wave']]), alpha=sp['alpha'], shape=(sp['M'], sp['N']),
+                        shift=sp['shift'], offset=sp['offset'])
+
+    def place(canvas, F, out):
+        r0, c0 = out
+        rs = slice(max(r0, 0), min(r0 + F.shape[0], shape[0]))
+        cs = slice(max(c0, 0), min(c0 + F.shape[1], shape[1]))
+        canvas[rs, cs] += F[rs.start - r0:rs.stop - r0, cs.start - c0:cs.stop - c0]
+
+    coherent = np.zeros(shape, complex)
+    for sp in specs[:2]:
+        place(coherent, field(sp), sp['out'])
+    ref = np.abs(coherent) ** 2
+    lone = np.zeros(shape)
+    place(lone, specs[2]['weight'] * np.abs(field(specs[2])) ** 2, specs[2]['out'])
+    ref += lone
+    print(f'mixed plan: {precision}', flush=True)
+    img = _raw_plan_image(tiles, wl, specs, shape, precision)
+    err = rel_l2(img, ref)
+    assert err < TOL_I, err
+    # the lone problem's window carries only its intensity (no fallback double count)
+    win = (slice(60, 78), slice(70, 80))
+    err = rel_l2(img[win], ref[win])
+    assert err < TOL_I, err
+
+
 # ------------------------------------------------------------------ error paths with a device
 
 def test_status_codes_with_a_context(L):
